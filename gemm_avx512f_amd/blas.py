@@ -1,0 +1,89 @@
+"""Python mirror of the reference's BLAS-3 interface, backed by libgemm_amd.so.
+
+Two layers, as in the C ABI:
+
+* ``sgemm_`` / ``dgemm_`` -- the Fortran-convention drop-in exactly as the
+  reference exports it (src/gemm_driver.c:116): every argument by reference,
+  host arrays, column-major, C updated in place.
+* ``sgemm`` / ``dgemm`` -- the same contract on device matrices
+  (:class:`DeviceMatrix`), stream-ordered.
+
+Nothing here computes: every call goes through the shared library, and a
+missing library raises.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceMatrix
+
+
+def _ref(x, ctype):
+    return ctypes.byref(ctype(x))
+
+
+def _char(c):
+    return ctypes.c_char_p(c.encode() if isinstance(c, str) else bytes(c))
+
+
+def _host_ptr(a, dtype):
+    if a.dtype != dtype:
+        raise TypeError("expected %s array, got %s" % (dtype, a.dtype))
+    if not a.flags.f_contiguous and not (a.ndim == 1 and a.flags.c_contiguous):
+        raise ValueError("host matrices must be Fortran-ordered (column-major)")
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _xgemm_(fn, ctype, dtype, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    fn(_char(transa), _char(transb), _ref(m, ctypes.c_int), _ref(n, ctypes.c_int), _ref(k, ctypes.c_int),
+       _ref(alpha, ctype), _host_ptr(a, dtype), _ref(lda, ctypes.c_int), _host_ptr(b, dtype),
+       _ref(ldb, ctypes.c_int), _ref(beta, ctype), _host_ptr(c, dtype), _ref(ldc, ctypes.c_int))
+
+
+def sgemm_(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """Fortran-ABI SGEMM on host arrays (C updated in place)."""
+    _xgemm_(_lib.lib().sgemm_, ctypes.c_float, np.float32, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta,
+            c, ldc)
+
+
+def dgemm_(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """Fortran-ABI DGEMM on host arrays (C updated in place)."""
+    _xgemm_(_lib.lib().dgemm_, ctypes.c_double, np.float64, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta,
+            c, ldc)
+
+
+def _stream(stream):
+    return stream.handle if stream is not None else None
+
+
+def _enc(c):
+    return c.encode() if isinstance(c, str) else c
+
+
+def sgemm(transa, transb, m, n, k, alpha, a: DeviceMatrix, b: DeviceMatrix, beta, c: DeviceMatrix, stream=None,
+          tile=None, splitk=None):
+    """C <- alpha*op(A)*op(B) + beta*C on device matrices (FP32)."""
+    L = _lib.lib()
+    if tile is None and splitk is None:
+        rc = L.gemm_amd_sgemm(_enc(transa), _enc(transb), m, n, k, alpha, a.ptr, a.ld, b.ptr, b.ld, beta, c.ptr,
+                              c.ld, _stream(stream))
+    else:
+        rc = L.gemm_amd_sgemm_cfg(_enc(transa), _enc(transb), m, n, k, alpha, a.ptr, a.ld, b.ptr, b.ld, beta, c.ptr,
+                                  c.ld, 0 if tile is None else tile, 1 if splitk is None else splitk,
+                                  _stream(stream))
+    return _lib.check(rc, "sgemm")
+
+
+def dgemm(transa, transb, m, n, k, alpha, a: DeviceMatrix, b: DeviceMatrix, beta, c: DeviceMatrix, stream=None,
+          tile=None, splitk=None):
+    """C <- alpha*op(A)*op(B) + beta*C on device matrices (FP64)."""
+    L = _lib.lib()
+    if tile is None and splitk is None:
+        rc = L.gemm_amd_dgemm(_enc(transa), _enc(transb), m, n, k, alpha, a.ptr, a.ld, b.ptr, b.ld, beta, c.ptr,
+                              c.ld, _stream(stream))
+    else:
+        rc = L.gemm_amd_dgemm_cfg(_enc(transa), _enc(transb), m, n, k, alpha, a.ptr, a.ld, b.ptr, b.ld, beta, c.ptr,
+                                  c.ld, 0 if tile is None else tile, 1 if splitk is None else splitk,
+                                  _stream(stream))
+    return _lib.check(rc, "dgemm")

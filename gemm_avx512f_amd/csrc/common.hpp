@@ -1,0 +1,53 @@
+// Shared device-side definitions for the gfx950 BLAS-3 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef double d2_t __attribute__((ext_vector_type(2)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef float f16_t __attribute__((ext_vector_type(16)));
+typedef int i4_t __attribute__((ext_vector_type(4)));
+typedef int i16_t __attribute__((ext_vector_type(16)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+
+template <typename T> struct Vec16;  // 16-byte vector of T
+template <> struct Vec16<double> { using t = d2_t; static constexpr int N = 2; };
+template <> struct Vec16<float> { using t = f4_t; static constexpr int N = 4; };
+
+// Lane layouts of the gfx950 matrix instructions used here (each one was
+// checked on hardware by one-hot / random-operand probes):
+//  v_mfma_f64_4x4x4_4b_f64: 4 independent blocks b of D_b(4x4) += A_b(4x4) B_b(4x4);
+//     A_b[i][k] at lane 4b+i+16k, B_b[k][j] at lane 4b+j+16k, D_b[i][j] at lane 16i+4b+j.
+//     (the 16x16x4 f64 form issues at ~60% of the FP64 peak on gfx950, this one at ~92%)
+//  v_mfma_f32_32x32x2_f32: A[i][k] lane i+32k, B[k][j] lane j+32k,
+//     D[i][j] lane j+32*((i/4)%2), element (i/8)*4 + i%4.
+//  v_mfma_f32_32x32x16_bf16 / v_mfma_i32_32x32x32_i8: lane l supplies 8 (bf16) or
+//     16 (i8) consecutive-k values of row i = l%32 (A) / column j = l%32 (B) for
+//     k-group l/32; D as for 32x32x2.
+__device__ __forceinline__ double mfma_f64_4x4x4(double a, double b, double c) {
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f16_t mfma_f32_32x32x2(float a, float b, f16_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+
+// Map a linear workgroup id to a tile id so that each of the 8 XCDs (which
+// receive workgroups round-robin: wg i runs on XCD i % 8) owns a contiguous
+// run of tiles, then order tiles in groups of `group` tile-rows so that
+// neighbouring tiles on one XCD share A row-panels and B column-panels in
+// that XCD's L2.
+__device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, int group,
+                                            int &tm, int &tn) {
+    const int ntiles = tiles_m * tiles_n;
+    int t = bid;
+    if ((ntiles & 7) == 0) t = (bid & 7) * (ntiles >> 3) + (bid >> 3);
+    const int per_group = group * tiles_n;
+    const int gid = t / per_group;
+    const int first_m = gid * group;
+    int gsize = tiles_m - first_m;
+    if (gsize > group) gsize = group;
+    const int r = t - gid * per_group;
+    tm = first_m + r % gsize;
+    tn = r / gsize;
+}

@@ -1,0 +1,477 @@
+// FP64 / FP32 GEMM on gfx950 matrix cores.
+//
+// Replaces the reference's blocked CPU GEMM (src/gemm_driver.c:116-195, its
+// packing in src/gemm_copy.c and the AVX-512 micro-kernels in
+// src/kernel/gemm_kernel.S:818-1213) with one kernel per (dtype, transA,
+// transB, tile shape):
+//   * one workgroup owns a BM x BN macro-tile of C (the reference's per-thread
+//     N chunk + BLOCK_DIM_M x BLOCK_DIM_N blocks become the grid);
+//   * op(A) and op(B) K-slabs are staged global -> registers -> LDS, double
+//     buffered, and transposed on the way in when the operand is stored
+//     M/N-contiguous (the reference's pack step, src/gemm_copy.c:4-199, done
+//     in-kernel); both LDS tiles are [row][k] with a 16-byte row pad so the
+//     16-byte fragment reads of 8 (16) consecutive rows hit distinct banks;
+//   * waves run 64x64 (or 32x32) register tiles of MFMA instructions;
+//   * the epilogue applies alpha and beta once per element (beta == 0 does not
+//     read C), the reference's C_MULT_BETA (src/gemm_driver.c:104-114) and
+//     alpha-folding into B (src/gemm_copy.c:64) fused;
+//   * split-K writes per-partition partial sums to a workspace and a second
+//     pass reduces them in a fixed order (deterministic).
+#include "common.hpp"
+#include "gemm_kernels.hpp"
+
+namespace gemm_amd {
+
+template <typename T>
+struct GemmParams {
+    int M, N, K;
+    T alpha, beta;
+    const T *A;
+    int64_t lda;
+    const T *B;
+    int64_t ldb;
+    T *C;
+    int64_t ldc;
+    int tiles_m, tiles_n;
+    int kchunk;   // K extent per split-K partition (multiple of BK)
+    T *work;      // split-K partials, [split][N][M], or nullptr
+};
+
+// ---------------------------------------------------------------- tile loader
+// Moves a ROWS x BK slab of an operand (row = M index of op(A) or N index of
+// op(B)) into LDS laid out [row][PITCH].  KCONTIG: the operand is stored with
+// k contiguous (element (r,k) at base[k + r*ld]); otherwise rows contiguous
+// (element (r,k) at base[r + k*ld]).  Elements outside (rlim, klim) read 0.
+template <typename T, int ROWS, int BK, int NT, bool KCONTIG, bool VEC>
+struct TileLoader {
+    using V = typename Vec16<T>::t;
+    static constexpr int VW = Vec16<T>::N;
+    static constexpr int PITCH = BK + VW;
+    static constexpr int NVEC = ROWS * BK / VW;
+    static_assert(NVEC % NT == 0, "tile must split evenly over the workgroup");
+    static constexpr int PER = NVEC / NT;
+    V reg[PER];
+
+    __device__ __forceinline__ static void coords(int v, int &r, int &k) {
+        if (KCONTIG) {
+            r = v / (BK / VW);
+            k = (v % (BK / VW)) * VW;
+        } else {
+            k = v / (ROWS / VW);
+            r = (v % (ROWS / VW)) * VW;
+        }
+    }
+
+    __device__ __forceinline__ void load(const T *__restrict__ base, int64_t ld, int rlim, int klim,
+                                         int tid) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            int r, k;
+            coords(tid + i * NT, r, k);
+            const T *src = KCONTIG ? base + (int64_t)r * ld + k : base + r + (int64_t)k * ld;
+            const bool full = KCONTIG ? (r < rlim && k + VW <= klim) : (r + VW <= rlim && k < klim);
+            if (VEC && full) {
+                reg[i] = *reinterpret_cast<const V *>(src);
+            } else {
+                V t;
+#pragma unroll
+                for (int e = 0; e < VW; ++e) {
+                    const bool ok = KCONTIG ? (r < rlim && k + e < klim) : (r + e < rlim && k < klim);
+                    t[e] = ok ? src[e] : T(0);
+                }
+                reg[i] = t;
+            }
+        }
+    }
+
+    __device__ __forceinline__ void store(T *lds, int tid) const {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            int r, k;
+            coords(tid + i * NT, r, k);
+            if (KCONTIG) {
+                *reinterpret_cast<V *>(lds + r * PITCH + k) = reg[i];
+            } else {
+#pragma unroll
+                for (int e = 0; e < VW; ++e) lds[(r + e) * PITCH + k] = reg[i][e];
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------- wave tiles
+// FP64: v_mfma_f64_4x4x4_4b.  A 16x16 sub-tile is four instructions: all use
+// the same A fragment (rows 4b..4b+3 of the slab in block b) and rotation r
+// feeds block b the B columns 4((b+r)%4)..+3, so the four rotations cover the
+// 16 column groups x 4 row groups once.  Lane group g = lane/16 owns
+// k = 4g..4g+3 of each 16-deep LDS slab (any k assignment is valid as long as
+// A and B use the same one).
+template <int WTM, int WTN, int BK>
+struct WaveF64 {
+    static constexpr int SM = WTM / 16, SN = WTN / 16;
+    static constexpr int PITCH = BK + 2;
+    static_assert(BK == 16, "FP64 wave tile expects a 16-deep slab");
+    double acc[SM][SN][4];
+    int aoff[SM], boff[SN][4];
+
+    __device__ __forceinline__ void init(int lane, int wm0, int wn0) {
+        const int q = lane & 15, g = lane >> 4, b = q >> 2, j = q & 3;
+#pragma unroll
+        for (int sm = 0; sm < SM; ++sm) aoff[sm] = (wm0 + sm * 16 + q) * PITCH + 4 * g;
+#pragma unroll
+        for (int sn = 0; sn < SN; ++sn)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) boff[sn][r] = (wn0 + sn * 16 + 4 * ((b + r) & 3) + j) * PITCH + 4 * g;
+#pragma unroll
+        for (int sm = 0; sm < SM; ++sm)
+#pragma unroll
+            for (int sn = 0; sn < SN; ++sn)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[sm][sn][r] = 0.0;
+    }
+
+    __device__ __forceinline__ void compute(const double *As, const double *Bs) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            d2_t av[SM], bv[SN][4];
+#pragma unroll
+            for (int sm = 0; sm < SM; ++sm) av[sm] = *reinterpret_cast<const d2_t *>(As + aoff[sm] + 2 * h);
+#pragma unroll
+            for (int sn = 0; sn < SN; ++sn)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) bv[sn][r] = *reinterpret_cast<const d2_t *>(Bs + boff[sn][r] + 2 * h);
+#pragma unroll
+            for (int e = 0; e < 2; ++e)
+#pragma unroll
+                for (int sm = 0; sm < SM; ++sm)
+#pragma unroll
+                    for (int sn = 0; sn < SN; ++sn)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            acc[sm][sn][r] = mfma_f64_4x4x4(av[sm][e], bv[sn][r][e], acc[sm][sn][r]);
+        }
+    }
+
+    // visit(row, col, value) for every accumulator element, tile-relative
+    template <typename F>
+    __device__ __forceinline__ void for_each(int lane, int wm0, int wn0, F &&visit) const {
+        const int i = lane >> 4, b = (lane >> 2) & 3, j = lane & 3;
+#pragma unroll
+        for (int sm = 0; sm < SM; ++sm)
+#pragma unroll
+            for (int sn = 0; sn < SN; ++sn)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    visit(wm0 + sm * 16 + 4 * b + i, wn0 + sn * 16 + 4 * ((b + r) & 3) + j, acc[sm][sn][r]);
+    }
+};
+
+// FP32: v_mfma_f32_32x32x2_f32.  Lane group g = lane/32 owns k = g*BK/2 ..
+// (g+1)*BK/2 - 1 of each slab, read 4 at a time.
+template <int WTM, int WTN, int BK>
+struct WaveF32 {
+    static constexpr int SM = WTM / 32, SN = WTN / 32;
+    static constexpr int PITCH = BK + 4;
+    static_assert(BK % 8 == 0, "FP32 slab depth must be a multiple of 8");
+    f16_t acc[SM][SN];
+    int aoff[SM], boff[SN];
+
+    __device__ __forceinline__ void init(int lane, int wm0, int wn0) {
+        const int q = lane & 31, g = lane >> 5;
+#pragma unroll
+        for (int sm = 0; sm < SM; ++sm) aoff[sm] = (wm0 + sm * 32 + q) * PITCH + (BK / 2) * g;
+#pragma unroll
+        for (int sn = 0; sn < SN; ++sn) boff[sn] = (wn0 + sn * 32 + q) * PITCH + (BK / 2) * g;
+#pragma unroll
+        for (int sm = 0; sm < SM; ++sm)
+#pragma unroll
+            for (int sn = 0; sn < SN; ++sn)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[sm][sn][e] = 0.0f;
+    }
+
+    __device__ __forceinline__ void compute(const float *As, const float *Bs) {
+#pragma unroll
+        for (int h = 0; h < BK / 8; ++h) {
+            f4_t av[SM], bv[SN];
+#pragma unroll
+            for (int sm = 0; sm < SM; ++sm) av[sm] = *reinterpret_cast<const f4_t *>(As + aoff[sm] + 4 * h);
+#pragma unroll
+            for (int sn = 0; sn < SN; ++sn) bv[sn] = *reinterpret_cast<const f4_t *>(Bs + boff[sn] + 4 * h);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+#pragma unroll
+                for (int sm = 0; sm < SM; ++sm)
+#pragma unroll
+                    for (int sn = 0; sn < SN; ++sn)
+                        acc[sm][sn] = mfma_f32_32x32x2(av[sm][e], bv[sn][e], acc[sm][sn]);
+        }
+    }
+
+    template <typename F>
+    __device__ __forceinline__ void for_each(int lane, int wm0, int wn0, F &&visit) const {
+        const int j = lane & 31, g = lane >> 5;
+#pragma unroll
+        for (int sm = 0; sm < SM; ++sm)
+#pragma unroll
+            for (int sn = 0; sn < SN; ++sn)
+#pragma unroll
+                for (int e = 0; e < 16; ++e)
+                    visit(wm0 + sm * 32 + (e >> 2) * 8 + g * 4 + (e & 3), wn0 + sn * 32 + j, acc[sm][sn][e]);
+    }
+};
+
+template <typename T, int WTM, int WTN, int BK> struct WaveSel;
+template <int WTM, int WTN, int BK> struct WaveSel<double, WTM, WTN, BK> { using type = WaveF64<WTM, WTN, BK>; };
+template <int WTM, int WTN, int BK> struct WaveSel<float, WTM, WTN, BK> { using type = WaveF32<WTM, WTN, BK>; };
+
+// ---------------------------------------------------------------- kernel
+template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, bool VEC>
+__global__ __launch_bounds__(WM *WN * 64) void gemm_mfma_kernel(GemmParams<T> p) {
+    constexpr int NT = WM * WN * 64;
+    constexpr int WTM = BM / WM, WTN = BN / WN;
+    using LA = TileLoader<T, BM, BK, NT, TA, VEC>;    // op(A) rows = M; k-contiguous iff A is transposed
+    using LB = TileLoader<T, BN, BK, NT, !TB, VEC>;   // op(B) rows = N; k-contiguous iff B is not transposed
+    using Wave = typename WaveSel<T, WTM, WTN, BK>::type;
+    constexpr int PITCH = LA::PITCH;
+    constexpr int STAGE = (BM + BN) * PITCH;
+
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *smem = reinterpret_cast<T *>(smem_raw);
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave % WM) * WTM, wn0 = (wave / WM) * WTN;
+
+    int tm, tn;
+    tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, 8, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int kbeg = blockIdx.y * p.kchunk;
+    const int kend = min(p.K, kbeg + p.kchunk);
+    const int mlim = p.M - m0, nlim = p.N - n0;
+
+    const T *abase = TA ? p.A + (int64_t)m0 * p.lda + kbeg : p.A + m0 + (int64_t)kbeg * p.lda;
+    const T *bbase = TB ? p.B + n0 + (int64_t)kbeg * p.ldb : p.B + (int64_t)n0 * p.ldb + kbeg;
+    const int64_t astep = TA ? BK : (int64_t)BK * p.lda;
+    const int64_t bstep = TB ? (int64_t)BK * p.ldb : BK;
+
+    LA la;
+    LB lb;
+    Wave w;
+    w.init(lane, wm0, wn0);
+
+    const int nk = (kend - kbeg + BK - 1) / BK;
+    if (nk > 0) {
+        la.load(abase, p.lda, mlim, kend - kbeg, tid);
+        lb.load(bbase, p.ldb, nlim, kend - kbeg, tid);
+        la.store(smem, tid);
+        lb.store(smem + BM * PITCH, tid);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        T *cur = smem + (kt & 1) * STAGE;
+        T *nxt = smem + ((kt + 1) & 1) * STAGE;
+        const bool more = kt + 1 < nk;
+        if (more) {
+            const int krem = kend - kbeg - (kt + 1) * BK;
+            la.load(abase + (kt + 1) * astep, p.lda, mlim, krem, tid);
+            lb.load(bbase + (kt + 1) * bstep, p.ldb, nlim, krem, tid);
+        }
+        w.compute(cur, cur + BM * PITCH);
+        if (more) {
+            la.store(nxt, tid);
+            lb.store(nxt + BM * PITCH, tid);
+        }
+        __syncthreads();
+    }
+
+    if (p.work) {
+        T *out = p.work + (int64_t)blockIdx.y * p.M * p.N;
+        w.for_each(lane, wm0, wn0, [&](int r, int c, T v) {
+            if (r < mlim && c < nlim) out[(m0 + r) + (int64_t)(n0 + c) * p.M] = v;
+        });
+    } else {
+        T *cbase = p.C + m0 + (int64_t)n0 * p.ldc;
+        const T alpha = p.alpha, beta = p.beta;
+        if (beta == T(0)) {
+            w.for_each(lane, wm0, wn0, [&](int r, int c, T v) {
+                if (r < mlim && c < nlim) cbase[r + (int64_t)c * p.ldc] = alpha * v;
+            });
+        } else {
+            w.for_each(lane, wm0, wn0, [&](int r, int c, T v) {
+                if (r < mlim && c < nlim) {
+                    T *dst = cbase + r + (int64_t)c * p.ldc;
+                    *dst = alpha * v + beta * *dst;
+                }
+            });
+        }
+    }
+}
+
+// Sum split-K partials in partition order, then apply alpha/beta.
+template <typename T>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(int M, int N, int splits, const T *__restrict__ work,
+                                                             T alpha, T beta, T *__restrict__ C, int64_t ldc) {
+    const int64_t total = (int64_t)M * N;
+    for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
+        T s = work[idx];
+        for (int z = 1; z < splits; ++z) s += work[idx + (int64_t)z * total];
+        const int i = (int)(idx % M);
+        const int64_t j = idx / M;
+        T *dst = C + i + j * ldc;
+        *dst = (beta == T(0)) ? alpha * s : alpha * s + beta * *dst;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void scale_kernel(int M, int N, T beta, T *__restrict__ C, int64_t ldc) {
+    const int64_t total = (int64_t)M * N;
+    for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
+        const int i = (int)(idx % M);
+        const int64_t j = idx / M;
+        T *dst = C + i + j * ldc;
+        *dst = (beta == T(0)) ? T(0) : beta * *dst;
+    }
+}
+
+// ---------------------------------------------------------------- host launch
+struct TileShape { int BM, BN, WM, WN; };
+// FP64 wave tiles hold 64 FP64 accumulators (128 VGPRs) at 64x64, so FP64
+// keeps one wave per SIMD (4-wave workgroups); FP32 affords 8-wave groups.
+static constexpr TileShape kTilesF64[TILE_COUNT] = {
+    {128, 128, 2, 2}, {64, 64, 2, 2}, {128, 64, 2, 2}, {256, 64, 4, 1}};
+static constexpr TileShape kTilesF32[TILE_COUNT] = {
+    {128, 128, 2, 2}, {64, 64, 2, 2}, {128, 64, 2, 2}, {256, 128, 4, 2}};
+
+static const TileShape &tile_shape(int t, int elem_bytes) {
+    return elem_bytes == 8 ? kTilesF64[t] : kTilesF32[t];
+}
+int tile_bm(int t, int elem_bytes) { return tile_shape(t, elem_bytes).BM; }
+int tile_bn(int t, int elem_bytes) { return tile_shape(t, elem_bytes).BN; }
+int tile_bk(int, int elem_bytes) { return elem_bytes == 8 ? 16 : 32; }
+
+size_t gemm_workspace_bytes(int m, int n, int, const GemmConfig &cfg, int elem_bytes) {
+    if (cfg.splitk <= 1) return 0;
+    return (size_t)cfg.splitk * (size_t)m * (size_t)n * (size_t)elem_bytes;
+}
+
+template <typename T, bool TA, bool TB, int BM, int BN, int WM, int WN, bool VEC>
+static hipError_t launch_one(const GemmParams<T> &p, int splits, hipStream_t s) {
+    constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+    constexpr int PITCH = BK + 16 / (int)sizeof(T);
+    constexpr size_t lds = 2ull * (BM + BN) * PITCH * sizeof(T);
+    auto kern = gemm_mfma_kernel<T, TA, TB, BM, BN, BK, WM, WN, VEC>;
+    static bool attr_set = false;   // per instantiation; set once per process
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    dim3 grid(p.tiles_m * p.tiles_n, splits);
+    hipLaunchKernelGGL(kern, grid, dim3(WM * WN * 64), lds, s, p);
+    return hipGetLastError();
+}
+
+template <typename T, bool TA, bool TB, bool VEC>
+static hipError_t launch_tile(int tile, const GemmParams<T> &p, int splits, hipStream_t s) {
+    switch (tile) {
+    case TILE_128x128: return launch_one<T, TA, TB, 128, 128, 2, 2, VEC>(p, splits, s);
+    case TILE_64x64: return launch_one<T, TA, TB, 64, 64, 2, 2, VEC>(p, splits, s);
+    case TILE_128x64: return launch_one<T, TA, TB, 128, 64, 2, 2, VEC>(p, splits, s);
+    case TILE_LARGE:
+        if constexpr (sizeof(T) == 8) return launch_one<T, TA, TB, 256, 64, 4, 1, VEC>(p, splits, s);
+        else return launch_one<T, TA, TB, 256, 128, 4, 2, VEC>(p, splits, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+static hipError_t launch_gemm_t(bool ta, bool tb, int m, int n, int k, T alpha, const T *A, int64_t lda,
+                                const T *B, int64_t ldb, T beta, T *C, int64_t ldc, const GemmConfig &cfg,
+                                void *work, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    if (k <= 0 || alpha == T(0)) {
+        if (beta == T(1)) return hipSuccess;
+        const int64_t total = (int64_t)m * n;
+        int blocks = (int)((total + 255) / 256);
+        if (blocks > 65536) blocks = 65536;
+        hipLaunchKernelGGL(scale_kernel<T>, dim3(blocks), dim3(256), 0, s, m, n, beta, C, ldc);
+        return hipGetLastError();
+    }
+    const int tile = (cfg.tile >= 0 && cfg.tile < TILE_COUNT) ? cfg.tile : TILE_128x128;
+    constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+    constexpr int VW = 16 / (int)sizeof(T);
+    GemmParams<T> p;
+    p.M = m; p.N = n; p.K = k;
+    p.alpha = alpha; p.beta = beta;
+    p.A = A; p.lda = lda; p.B = B; p.ldb = ldb; p.C = C; p.ldc = ldc;
+    const TileShape &ts = tile_shape(tile, (int)sizeof(T));
+    p.tiles_m = (m + ts.BM - 1) / ts.BM;
+    p.tiles_n = (n + ts.BN - 1) / ts.BN;
+    int splits = cfg.splitk > 1 ? cfg.splitk : 1;
+    int kchunk = (k + splits - 1) / splits;
+    kchunk = (kchunk + BK - 1) / BK * BK;
+    splits = (k + kchunk - 1) / kchunk;
+    p.kchunk = kchunk;
+    p.work = (splits > 1) ? static_cast<T *>(work) : nullptr;
+    if (splits > 1 && !work) return hipErrorInvalidValue;
+
+    const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && (lda % VW == 0) &&
+                     (ldb % VW == 0);
+    hipError_t e;
+    const int sel = (ta ? 2 : 0) | (tb ? 1 : 0);
+    if (vec) {
+        switch (sel) {
+        case 0: e = launch_tile<T, false, false, true>(tile, p, splits, s); break;
+        case 1: e = launch_tile<T, false, true, true>(tile, p, splits, s); break;
+        case 2: e = launch_tile<T, true, false, true>(tile, p, splits, s); break;
+        default: e = launch_tile<T, true, true, true>(tile, p, splits, s); break;
+        }
+    } else {
+        switch (sel) {
+        case 0: e = launch_tile<T, false, false, false>(tile, p, splits, s); break;
+        case 1: e = launch_tile<T, false, true, false>(tile, p, splits, s); break;
+        case 2: e = launch_tile<T, true, false, false>(tile, p, splits, s); break;
+        default: e = launch_tile<T, true, true, false>(tile, p, splits, s); break;
+        }
+    }
+    if (e != hipSuccess || splits == 1) return e;
+    const int64_t total = (int64_t)m * n;
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(splitk_reduce_kernel<T>, dim3(blocks), dim3(256), 0, s, m, n, splits,
+                       static_cast<const T *>(work), alpha, beta, C, ldc);
+    return hipGetLastError();
+}
+
+int launch_dgemm(bool ta, bool tb, int m, int n, int k, double alpha, const double *A, int64_t lda,
+                 const double *B, int64_t ldb, double beta, double *C, int64_t ldc, const GemmConfig &cfg,
+                 void *work, hipStream_t s) {
+    return (int)launch_gemm_t<double>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, cfg, work, s);
+}
+
+int launch_sgemm(bool ta, bool tb, int m, int n, int k, float alpha, const float *A, int64_t lda,
+                 const float *B, int64_t ldb, float beta, float *C, int64_t ldc, const GemmConfig &cfg,
+                 void *work, hipStream_t s) {
+    return (int)launch_gemm_t<float>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, cfg, work, s);
+}
+
+int launch_scale_d(int m, int n, double beta, double *C, int64_t ldc, hipStream_t s) {
+    if (m <= 0 || n <= 0) return 0;
+    const int64_t total = (int64_t)m * n;
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(scale_kernel<double>, dim3(blocks), dim3(256), 0, s, m, n, beta, C, ldc);
+    return (int)hipGetLastError();
+}
+
+int launch_scale_s(int m, int n, float beta, float *C, int64_t ldc, hipStream_t s) {
+    if (m <= 0 || n <= 0) return 0;
+    const int64_t total = (int64_t)m * n;
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(scale_kernel<float>, dim3(blocks), dim3(256), 0, s, m, n, beta, C, ldc);
+    return (int)hipGetLastError();
+}
+
+}  // namespace gemm_amd

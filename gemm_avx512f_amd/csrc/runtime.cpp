@@ -1,0 +1,198 @@
+// Host runtime of libgemm_amd.so: device state, buffer cache, errors, and
+// the plumbing entry points (memory, copies, streams, events).
+#include "runtime.hpp"
+#include "../../include/gemm_amd.h"
+#include <cstdio>
+#include <cstring>
+
+namespace gemm_amd {
+
+static int g_init_status = 1;   // 1 = not yet initialised
+static Runtime g_rt;
+static std::once_flag g_once;
+
+static void do_init() {
+    hipError_t e = hipGetDevice(&g_rt.device);
+    if (e == hipSuccess) {
+        hipDeviceProp_t prop;
+        e = hipGetDeviceProperties(&prop, g_rt.device);
+        if (e == hipSuccess) g_rt.cus = prop.multiProcessorCount;
+    }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking);
+    g_init_status = (e == hipSuccess) ? 0 : -(int)e;
+    if (e != hipSuccess) {
+        std::string m = "gemm_amd: HIP initialisation failed: ";
+        m += hipGetErrorString(e);
+        set_error(m);
+    }
+}
+
+Runtime &rt() {
+    std::call_once(g_once, do_init);
+    return g_rt;
+}
+
+int init_status() {
+    rt();
+    return g_init_status;
+}
+
+static thread_local std::string t_err;
+void set_error(const std::string &msg) { t_err = msg; }
+const char *get_error() { return t_err.c_str(); }
+
+int hip_status(hipError_t err, const char *where) {
+    if (err == hipSuccess) return 0;
+    std::string m = "gemm_amd: ";
+    m += where;
+    m += ": ";
+    m += hipGetErrorString(err);
+    set_error(m);
+    return -(int)err;
+}
+
+int DevBuf::ensure(size_t need) {
+    if (need <= bytes && ptr) return 0;
+    if (ptr) {
+        hipError_t e = hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        if (e != hipSuccess) return hip_status(e, "hipFree");
+    }
+    size_t alloc = need < 256 ? 256 : need;
+    alloc = (alloc + 0xFFFFF) & ~(size_t)0xFFFFF;   // round up to 1 MiB
+    hipError_t e = hipMalloc(&ptr, alloc);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        return hip_status(e, "hipMalloc (buffer cache)");
+    }
+    bytes = alloc;
+    return 0;
+}
+
+static DevBuf g_bufs[BUF_COUNT];
+DevBuf &buf(BufId id) { return g_bufs[id]; }
+
+static std::mutex g_ws_mu;
+static DevBuf g_ws;
+void *device_workspace(size_t bytes, int *status) {
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    int st = 0;
+    if (bytes > g_ws.bytes) {
+        // growth must not free memory another stream may still be using
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) st = hip_status(e, "hipDeviceSynchronize (workspace growth)");
+        if (st == 0) st = g_ws.ensure(bytes);
+    }
+    if (status) *status = st;
+    return st == 0 ? g_ws.ptr : nullptr;
+}
+
+void xerbla(const char *name, int info) {
+    char msg[160];
+    snprintf(msg, sizeof msg, " ** On entry to %-6s parameter number %2d had an illegal value", name, info);
+    fprintf(stderr, "%s\n", msg);
+    set_error(msg);
+}
+
+int gemm_check(char ta, char tb, int m, int n, int k, int lda, int ldb, int ldc) {
+    const int nrowa = notrans(ta) ? m : k;
+    const int nrowb = notrans(tb) ? k : n;
+    if (!is_trans_char(ta)) return 1;
+    if (!is_trans_char(tb)) return 2;
+    if (m < 0) return 3;
+    if (n < 0) return 4;
+    if (k < 0) return 5;
+    if (lda < (nrowa > 1 ? nrowa : 1)) return 8;
+    if (ldb < (nrowb > 1 ? nrowb : 1)) return 10;
+    if (ldc < (m > 1 ? m : 1)) return 13;
+    return 0;
+}
+
+}  // namespace gemm_amd
+
+using namespace gemm_amd;
+
+extern "C" {
+
+int gemm_amd_init(void) { return init_status(); }
+
+int gemm_amd_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int gemm_amd_device_info(int64_t *info) {
+    if (int st = init_status()) return st;
+    hipDeviceProp_t p;
+    hipError_t e = hipGetDeviceProperties(&p, rt().device);
+    if (e != hipSuccess) return hip_status(e, "hipGetDeviceProperties");
+    info[0] = p.multiProcessorCount;
+    info[1] = p.clockRate;
+    info[2] = (int64_t)p.sharedMemPerBlock;
+    info[3] = p.l2CacheSize;
+    return 0;
+}
+
+const char *gemm_amd_last_error(void) { return get_error(); }
+
+void *gemm_amd_malloc(size_t bytes) {
+    if (init_status()) return nullptr;
+    void *p = nullptr;
+    if (hip_status(hipMalloc(&p, bytes ? bytes : 1), "hipMalloc")) return nullptr;
+    return p;
+}
+
+int gemm_amd_free(void *p) { return hip_status(hipFree(p), "hipFree"); }
+
+int gemm_amd_memcpy(void *dst, const void *src, size_t bytes, int kind) {
+    if (int st = init_status()) return st;
+    hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost
+                                                                     : hipMemcpyDeviceToDevice;
+    return hip_status(hipMemcpy(dst, src, bytes, k), "hipMemcpy");
+}
+
+int gemm_amd_memset(void *p, int value, size_t bytes) {
+    if (int st = init_status()) return st;
+    return hip_status(hipMemset(p, value, bytes), "hipMemset");
+}
+
+int gemm_amd_synchronize(void) {
+    if (int st = init_status()) return st;
+    return hip_status(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+
+void *gemm_amd_stream_create(void) {
+    if (init_status()) return nullptr;
+    hipStream_t s = nullptr;
+    if (hip_status(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate")) return nullptr;
+    return (void *)s;
+}
+
+int gemm_amd_stream_destroy(void *s) { return hip_status(hipStreamDestroy((hipStream_t)s), "hipStreamDestroy"); }
+
+int gemm_amd_stream_synchronize(void *s) {
+    return hip_status(hipStreamSynchronize((hipStream_t)s), "hipStreamSynchronize");
+}
+
+void *gemm_amd_event_create(void) {
+    if (init_status()) return nullptr;
+    hipEvent_t e = nullptr;
+    if (hip_status(hipEventCreate(&e), "hipEventCreate")) return nullptr;
+    return (void *)e;
+}
+
+int gemm_amd_event_destroy(void *e) { return hip_status(hipEventDestroy((hipEvent_t)e), "hipEventDestroy"); }
+
+int gemm_amd_event_record(void *e, void *s) {
+    return hip_status(hipEventRecord((hipEvent_t)e, (hipStream_t)s), "hipEventRecord");
+}
+
+int gemm_amd_event_elapsed_ms(void *a, void *b, float *ms) {
+    hipError_t e = hipEventSynchronize((hipEvent_t)b);
+    if (e != hipSuccess) return hip_status(e, "hipEventSynchronize");
+    return hip_status(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b), "hipEventElapsedTime");
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+// Host runtime shared by the C-ABI entry points: device state, buffer cache,
+// error reporting and argument checking.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <mutex>
+
+namespace gemm_amd {
+
+struct Runtime {
+    int device = 0;
+    int cus = 256;
+    hipStream_t stream = nullptr;   // the library's stream for host-pointer calls
+    std::mutex mu;                  // serialises host-pointer calls and the buffer cache
+};
+
+Runtime &rt();                       // initialises on first use (throws nothing; check init_status)
+int init_status();                   // 0 or negative hipError_t of the first init
+
+// Thread-local last-error message.
+void set_error(const std::string &msg);
+const char *get_error();
+
+// Returns 0, or -(int)err after recording the message.
+int hip_status(hipError_t err, const char *where);
+
+// A growable device buffer owned by the library (never shrinks).
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);         // 0 or negative HIP error
+};
+
+// Named scratch buffers: host-pointer staging (A, B, C) and kernel workspaces.
+enum BufId { BUF_A = 0, BUF_B, BUF_C, BUF_WORK, BUF_WORK2, BUF_WORK3, BUF_COUNT };
+DevBuf &buf(BufId id);                     // for host-pointer calls (under rt().mu)
+// Workspace for device-pointer calls; guarded by its own mutex so concurrent
+// device calls on different streams serialise only on growth.
+void *device_workspace(size_t bytes, int *status);
+
+// netlib xerbla message: " ** On entry to NAME parameter number I had an illegal value"
+void xerbla(const char *name, int info);
+
+inline bool lsame(char a, char b) { return a == b || a == (char)(b + 32); }
+inline bool is_trans_char(char t) { return lsame(t, 'N') || lsame(t, 'T') || lsame(t, 'C'); }
+inline bool notrans(char t) { return t == 'N' || t == 'n'; }
+
+// Checks the GEMM argument list exactly as reference BLAS does (info = index of
+// the first illegal argument in the Fortran argument list, 0 if all legal).
+int gemm_check(char ta, char tb, int m, int n, int k, int lda, int ldb, int ldc);
+
+}  // namespace gemm_amd

@@ -1,0 +1,100 @@
+/*
+ * gemm_amd.h -- C ABI of libgemm_amd.so, an MI355X (gfx950) BLAS-3 library.
+ *
+ * Part 1 is the drop-in: the Fortran-convention, column-major entry points
+ * of the reference (wjc404/GEMM_AVX512F) with the same argument meaning.
+ * They take HOST pointers; the library copies operands to HBM, runs the HIP
+ * kernels on one GPU and copies C back before returning.
+ *
+ * Part 2 is the device-pointer, stream-ordered variant of every entry point
+ * (arguments by value, pointers into device memory, work enqueued on
+ * `stream`, NULL = the null stream).  They return 0 on success, a positive
+ * netlib-style parameter index when an argument is illegal (the same index
+ * xerbla would print), or a negative HIP error code.
+ *
+ * Part 3 is runtime plumbing (device memory, copies, events) so that a C or
+ * Python caller needs nothing but this library.
+ *
+ * Integer arguments are 32-bit (LP64), as in the reference.  Complex values
+ * are interleaved (re, im) pairs; complex leading dimensions count complex
+ * elements.
+ */
+#ifndef GEMM_AMD_H
+#define GEMM_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================================================== Part 1: drop-in (host pointers) */
+
+/* C <- alpha*op(A)*op(B) + beta*C, op(X) = X for 'N'/'n', X^T otherwise.
+ * Replaces sgemm_ of SGEMM.so (src/gemm_driver.c:116, float build; Makefile:14-15).
+ * beta == 0 overwrites C without reading it (netlib); the reference multiplies
+ * by zero instead (src/gemm_driver.c:110) -- identical for finite C. */
+void sgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+            const float *alpha, const float *a, const int *lda, const float *b, const int *ldb,
+            const float *beta, float *c, const int *ldc);
+
+/* Replaces dgemm_ of DGEMM.so (src/gemm_driver.c:116 with -DDOUBLE; Makefile:11-12). */
+void dgemm_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+            const double *alpha, const double *a, const int *lda, const double *b, const int *ldb,
+            const double *beta, double *c, const int *ldc);
+
+/* ===================================================== Part 2: device pointers */
+
+int gemm_amd_sgemm(char transa, char transb, int m, int n, int k, float alpha,
+                   const float *a, int lda, const float *b, int ldb, float beta,
+                   float *c, int ldc, void *stream);
+int gemm_amd_dgemm(char transa, char transb, int m, int n, int k, double alpha,
+                   const double *a, int lda, const double *b, int ldb, double beta,
+                   double *c, int ldc, void *stream);
+
+/* Same products with an explicit kernel configuration (tile id 0..3 =
+ * 128x128, 64x64, 128x64, and 256x64 (FP64) or 256x128 (FP32); splitk >= 1), bypassing the picker.
+ * Used by the autotuner; the result is identical up to summation order. */
+int gemm_amd_sgemm_cfg(char transa, char transb, int m, int n, int k, float alpha,
+                       const float *a, int lda, const float *b, int ldb, float beta,
+                       float *c, int ldc, int tile, int splitk, void *stream);
+int gemm_amd_dgemm_cfg(char transa, char transb, int m, int n, int k, double alpha,
+                       const double *a, int lda, const double *b, int ldb, double beta,
+                       double *c, int ldc, int tile, int splitk, void *stream);
+
+/* ===================================================== Part 3: runtime */
+
+int gemm_amd_init(void);                         /* 0 or negative HIP error; idempotent */
+int gemm_amd_device_count(void);
+/* fills: [0]=compute units, [1]=max clock kHz, [2]=LDS bytes per workgroup, [3]=L2 bytes */
+int gemm_amd_device_info(int64_t *info4);
+const char *gemm_amd_last_error(void);            /* message of the last failure on this thread */
+
+void *gemm_amd_malloc(size_t bytes);              /* device memory, NULL on failure */
+int gemm_amd_free(void *dptr);
+/* kind: 1 host->device, 2 device->host, 3 device->device; synchronous */
+int gemm_amd_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int gemm_amd_memset(void *dptr, int value, size_t bytes);
+int gemm_amd_synchronize(void);
+
+void *gemm_amd_stream_create(void);               /* default-priority stream */
+int gemm_amd_stream_destroy(void *stream);
+int gemm_amd_stream_synchronize(void *stream);
+void *gemm_amd_event_create(void);
+int gemm_amd_event_destroy(void *event);
+int gemm_amd_event_record(void *event, void *stream);
+int gemm_amd_event_elapsed_ms(void *start, void *stop, float *ms);   /* waits for stop */
+
+/* Seeded synthetic input in device memory (no reference counterpart: the
+ * benchmark and parity inputs).  Element (i, j) of the rows x cols column-major
+ * matrix gets draw number first_draw + j*rows + i of the splitmix64 stream
+ * seeded with `seed` (two draws per element for kind 2).  kind 0: FP64
+ * uniform[-1,1) = ((x>>11)*2^-53)*2-1; kind 1: FP32 ((x>>40)*2^-24)*2-1;
+ * kind 2: FP64 u*2^e, e uniform integer in [-range_r, range_r].  Padding rows
+ * (i >= rows) are not written. */
+int gemm_amd_fill_uniform(void *dptr, int kind, int64_t rows, int64_t cols, int64_t ld, uint64_t seed,
+                          uint64_t first_draw, int range_r, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GEMM_AMD_H */

@@ -1,0 +1,97 @@
+"""The C-ABI library: it loads, exports every symbol the headers declare, and
+its host-side argument checking behaves like reference BLAS -- all without a
+GPU (argument checks run before any device work)."""
+import ctypes
+import glob
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from gemm_avx512f_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(", text, re.M):
+            names.add(m.group(1))
+    names.discard("extern")
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    syms = declared_symbols()
+    assert "dgemm_" in syms and "sgemm_" in syms
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    # and the Python binding covers exactly the declared ABI
+    assert set(syms) == set(_lib.SIGNATURES), set(syms) ^ set(_lib.SIGNATURES)
+
+
+def test_exported_dynamic_symbols_are_only_the_abi():
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    text_syms = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    ours = {s for s in text_syms if s.endswith("_") and s[:-1].isalpha() or s.startswith("gemm_amd_")}
+    assert set(declared_symbols()) <= text_syms
+    assert ours <= set(declared_symbols()) | {s for s in text_syms if not s.startswith("gemm_amd_")}
+
+
+@pytest.mark.parametrize("args,info", [
+    (("X", "N", 4, 4, 4, 4, 4, 4), 1),
+    (("N", "Q", 4, 4, 4, 4, 4, 4), 2),
+    (("N", "N", -1, 4, 4, 4, 4, 4), 3),
+    (("N", "N", 4, -1, 4, 4, 4, 4), 4),
+    (("N", "N", 4, 4, -1, 4, 4, 4), 5),
+    (("N", "N", 4, 4, 4, 3, 4, 4), 8),
+    (("T", "N", 4, 4, 5, 4, 5, 4), 8),     # A stored K x M: lda >= K
+    (("N", "N", 4, 4, 5, 4, 4, 4), 10),
+    (("N", "T", 4, 6, 5, 4, 5, 4), 10),    # B stored N x K: ldb >= N
+    (("N", "N", 4, 4, 4, 4, 4, 3), 13),
+])
+def test_device_entry_argument_checks(args, info, capfd):
+    ta, tb, m, n, k, lda, ldb, ldc = args
+    L = _lib.lib()
+    for fn in (L.gemm_amd_dgemm, L.gemm_amd_sgemm):
+        rc = fn(ta.encode(), tb.encode(), m, n, k, 1.0, None, lda, None, ldb, 0.0, None, ldc, None)
+        assert rc == info
+    err = capfd.readouterr().err
+    assert "parameter number %2d had an illegal value" % info in err
+    assert "DGEMM" in err and "SGEMM" in err
+
+
+def test_host_entry_rejects_and_leaves_c_untouched(capfd):
+    from gemm_avx512f_amd import blas
+    c = np.asfortranarray(np.full((4, 4), 7.0))
+    a = np.asfortranarray(np.zeros((4, 4)))
+    blas.dgemm_("N", "N", 4, 4, 4, 1.0, a, 3, a, 4, 0.0, c, 4)   # lda < m
+    assert np.all(c == 7.0)
+    assert "DGEMM  parameter number  8" in capfd.readouterr().err
+
+
+def test_quick_returns_need_no_device():
+    L = _lib.lib()
+    # m == 0, n == 0, and (alpha == 0 or k == 0) with beta == 1 do nothing and succeed
+    assert L.gemm_amd_dgemm(b"N", b"N", 0, 5, 5, 1.0, None, 1, None, 5, 0.0, None, 1, None) == 0
+    assert L.gemm_amd_dgemm(b"N", b"N", 5, 0, 5, 1.0, None, 5, None, 5, 0.0, None, 5, None) == 0
+    assert L.gemm_amd_sgemm(b"N", b"N", 5, 5, 0, 1.0, None, 5, None, 1, 1.0, None, 5, None) == 0
+    assert L.gemm_amd_sgemm(b"T", b"T", 5, 5, 5, 0.0, None, 5, None, 5, 1.0, None, 5, None) == 0
+    c = np.asfortranarray(np.full((3, 3), 2.0))
+    from gemm_avx512f_amd import blas
+    blas.dgemm_("N", "N", 3, 3, 0, 1.0, c, 3, c, 1, 1.0, c, 3)
+    assert np.all(c == 2.0)
+
+
+def test_python_binding_fails_loudly_without_library(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
+    with pytest.raises(_lib.GemmAmdError):
+        _lib.lib()
