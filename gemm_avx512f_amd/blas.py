@@ -87,3 +87,62 @@ def dgemm(transa, transb, m, n, k, alpha, a: DeviceMatrix, b: DeviceMatrix, beta
                                   c.ld, 0 if tile is None else tile, 1 if splitk is None else splitk,
                                   _stream(stream))
     return _lib.check(rc, "dgemm")
+
+
+# ------------------------------------------------------------------ DTRSM
+def dtrsm_(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb):
+    """Fortran-ABI DTRSM on host arrays (B overwritten with the solution)."""
+    _lib.lib().dtrsm_(_char(side), _char(uplo), _char(transa), _char(diag), _ref(m, ctypes.c_int),
+                      _ref(n, ctypes.c_int), _ref(alpha, ctypes.c_double), _host_ptr(a, np.float64),
+                      _ref(lda, ctypes.c_int), _host_ptr(b, np.float64), _ref(ldb, ctypes.c_int))
+
+
+def dtrsm(side, uplo, transa, diag, m, n, alpha, a: DeviceMatrix, b: DeviceMatrix, stream=None):
+    """B <- alpha*op(A)^-1*B (side L) or alpha*B*op(A)^-1 (side R) on device matrices."""
+    rc = _lib.lib().gemm_amd_dtrsm(_enc(side), _enc(uplo), _enc(transa), _enc(diag), m, n, alpha, a.ptr, a.ld, b.ptr,
+                                   b.ld, _stream(stream))
+    return _lib.check(rc, "dtrsm")
+
+
+TRSM_KERNEL_VARIANTS = {"ln": 0, "lt": 1, "rn": 2, "rt": 3}
+
+
+def dtrsm_kernel(variant, M, N, sa, sb, c, LDB, LDC, stream=None):
+    """Kernel-level solve (reference dtrsm_solve_<variant>_m<M>n<N> contract) on device buffers."""
+    v = TRSM_KERNEL_VARIANTS[variant] if isinstance(variant, str) else int(variant)
+    rc = _lib.lib().gemm_amd_dtrsm_kernel(v, M, N, sa.ptr, sb.ptr, c.ptr, LDB, LDC, _stream(stream))
+    return _lib.check(rc, "dtrsm_kernel")
+
+
+# ------------------------------------------------------------------ 3M complex GEMM
+def _cpair(x, ctype):
+    x = complex(x)
+    return (ctype * 2)(x.real, x.imag)
+
+
+def zgemm3m_(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """Fortran-ABI ZGEMM3M on host complex128 arrays (C updated in place)."""
+    _lib.lib().zgemm3m_(_char(transa), _char(transb), _ref(m, ctypes.c_int), _ref(n, ctypes.c_int),
+                        _ref(k, ctypes.c_int), _cpair(alpha, ctypes.c_double), _host_ptr(a, np.complex128),
+                        _ref(lda, ctypes.c_int), _host_ptr(b, np.complex128), _ref(ldb, ctypes.c_int),
+                        _cpair(beta, ctypes.c_double), _host_ptr(c, np.complex128), _ref(ldc, ctypes.c_int))
+
+
+def cgemm3m_(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """Fortran-ABI CGEMM3M on host complex64 arrays (C updated in place)."""
+    _lib.lib().cgemm3m_(_char(transa), _char(transb), _ref(m, ctypes.c_int), _ref(n, ctypes.c_int),
+                        _ref(k, ctypes.c_int), _cpair(alpha, ctypes.c_float), _host_ptr(a, np.complex64),
+                        _ref(lda, ctypes.c_int), _host_ptr(b, np.complex64), _ref(ldb, ctypes.c_int),
+                        _cpair(beta, ctypes.c_float), _host_ptr(c, np.complex64), _ref(ldc, ctypes.c_int))
+
+
+def zgemm3m(transa, transb, m, n, k, alpha, a: DeviceMatrix, b: DeviceMatrix, beta, c: DeviceMatrix, stream=None):
+    rc = _lib.lib().gemm_amd_zgemm3m(_enc(transa), _enc(transb), m, n, k, _cpair(alpha, ctypes.c_double), a.ptr, a.ld,
+                                     b.ptr, b.ld, _cpair(beta, ctypes.c_double), c.ptr, c.ld, _stream(stream))
+    return _lib.check(rc, "zgemm3m")
+
+
+def cgemm3m(transa, transb, m, n, k, alpha, a: DeviceMatrix, b: DeviceMatrix, beta, c: DeviceMatrix, stream=None):
+    rc = _lib.lib().gemm_amd_cgemm3m(_enc(transa), _enc(transb), m, n, k, _cpair(alpha, ctypes.c_float), a.ptr, a.ld,
+                                     b.ptr, b.ld, _cpair(beta, ctypes.c_float), c.ptr, c.ld, _stream(stream))
+    return _lib.check(rc, "cgemm3m")

@@ -42,7 +42,7 @@ int gemm_device(const char *name, char ta, char tb, int m, int n, int k, T alpha
     void *work = nullptr;
     if (wbytes) {
         int st = 0;
-        work = device_workspace(wbytes, &st);
+        work = device_workspace(WS_GEMM, wbytes, &st);
         if (st) return st;
     }
     int e = launch<T>(!notrans(ta), !notrans(tb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, cfg, work, s);

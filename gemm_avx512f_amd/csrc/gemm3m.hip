@@ -1,0 +1,156 @@
+// Complex GEMM by the 3M method (cgemm3m_/zgemm3m_) on gfx950.
+//
+// The reference's 3M kernels compute one real product T of packed real
+// planes and accumulate C += (alpha_r + i alpha_i) * T into interleaved
+// complex C (cgemm3m_kernel_16x4_skylakex.c:41-58, 416-440;
+// zgemm3m_kernel_2x8_skylakex.c:336-351); OpenBLAS's gemm3m driver (not in the
+// reference) calls them three times on the planes Ar, Ai, Ar+Ai.  Here:
+//   1. one pass splits op(A) and op(B) (transpose / conjugate applied) into the
+//      real planes Xr, Xi, Xr+Xi, column-major without padding;
+//   2. three real GEMMs on the FP32/FP64 matrix-core kernel:
+//      T1 = Ar Br, T2 = Ai Bi, T3 = (Ar+Ai)(Br+Bi);
+//   3. one pass combines Re = T1 - T2, Im = T3 - T1 - T2 and applies the
+//      complex alpha and beta (beta == 0 does not read C).
+#include "common.hpp"
+#include "gemm_kernels.hpp"
+#include "gemm3m.hpp"
+
+namespace gemm_amd {
+
+// planes[0..2] of op(X) (rows x cols, ld rows): re, im, re+im.  op = N, T or
+// C (conjugate transpose).  X is interleaved complex with leading dim ldx.
+template <typename T>
+__global__ __launch_bounds__(256) void split_planes_kernel(int rows, int cols, const T *__restrict__ X, int64_t ldx,
+                                                           bool trans, bool conj, T *__restrict__ planes) {
+    const int64_t total = (int64_t)rows * cols;
+    for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
+        const int64_t r = idx % rows, c = idx / rows;
+        const int64_t src = trans ? (c + r * ldx) : (r + c * ldx);
+        const T re = X[2 * src];
+        T im = X[2 * src + 1];
+        if (conj) im = -im;
+        planes[idx] = re;
+        planes[idx + total] = im;
+        planes[idx + 2 * total] = re + im;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void combine3m_kernel(int m, int n, const T *__restrict__ t, T ar, T ai, T br, T bi,
+                                                        T *__restrict__ C, int64_t ldc) {
+    const int64_t total = (int64_t)m * n;
+    for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
+        const int64_t i = idx % m, j = idx / m;
+        const T t1 = t[idx], t2 = t[idx + total], t3 = t[idx + 2 * total];
+        const T re = t1 - t2, im = t3 - t1 - t2;
+        T *c = C + 2 * (i + j * ldc);
+        T cr = ar * re - ai * im, ci = ar * im + ai * re;
+        if (br != T(0) || bi != T(0)) {
+            const T xr = c[0], xi = c[1];
+            cr += br * xr - bi * xi;
+            ci += br * xi + bi * xr;
+        }
+        c[0] = cr;
+        c[1] = ci;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void cscale_kernel(int m, int n, T br, T bi, T *__restrict__ C, int64_t ldc) {
+    const int64_t total = (int64_t)m * n;
+    for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
+        const int64_t i = idx % m, j = idx / m;
+        T *c = C + 2 * (i + j * ldc);
+        if (br == T(0) && bi == T(0)) {
+            c[0] = T(0);
+            c[1] = T(0);
+        } else {
+            const T xr = c[0], xi = c[1];
+            c[0] = br * xr - bi * xi;
+            c[1] = br * xi + bi * xr;
+        }
+    }
+}
+
+static int grid_of(int64_t total) {
+    int64_t b = (total + 255) / 256;
+    return (int)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+}
+
+size_t gemm3m_workspace_bytes(int m, int n, int k, int elem_bytes) {
+    return (size_t)elem_bytes * 3 * ((size_t)m * k + (size_t)k * n + (size_t)m * n);
+}
+
+template <typename T>
+static int gemm3m_t(char ta, char tb, int m, int n, int k, const T *alpha, const T *A, int64_t lda, const T *B,
+                    int64_t ldb, const T *beta, T *C, int64_t ldc, void *ws, void *gemm_ws, size_t gemm_ws_bytes,
+                    int cus, hipStream_t s) {
+    if (m <= 0 || n <= 0) return 0;
+    const T ar = alpha[0], ai = alpha[1], br = beta[0], bi = beta[1];
+    if (k <= 0 || (ar == T(0) && ai == T(0))) {
+        if (br == T(1) && bi == T(0)) return 0;
+        hipLaunchKernelGGL(cscale_kernel<T>, dim3(grid_of((int64_t)m * n)), dim3(256), 0, s, m, n, br, bi, C, ldc);
+        return (int)hipGetLastError();
+    }
+    T *pa = static_cast<T *>(ws);
+    T *pb = pa + 3 * (size_t)m * k;
+    T *pt = pb + 3 * (size_t)k * n;
+    const bool tA = !(ta == 'N' || ta == 'n'), cA = (ta == 'C' || ta == 'c');
+    const bool tB = !(tb == 'N' || tb == 'n'), cB = (tb == 'C' || tb == 'c');
+    hipLaunchKernelGGL(split_planes_kernel<T>, dim3(grid_of((int64_t)m * k)), dim3(256), 0, s, m, k, A, lda, tA, cA,
+                       pa);
+    hipLaunchKernelGGL(split_planes_kernel<T>, dim3(grid_of((int64_t)k * n)), dim3(256), 0, s, k, n, B, ldb, tB, cB,
+                       pb);
+    if (hipError_t e = hipGetLastError()) return (int)e;
+    const size_t mk = (size_t)m * k, kn = (size_t)k * n, mn = (size_t)m * n;
+    const int dt = sizeof(T) == 8 ? 1 : 0;
+    for (int p = 0; p < 3; ++p) {
+        GemmConfig cfg = pick_complex_gemm_config(dt, m, n, k, cus);
+        if (gemm_workspace_bytes(m, n, k, cfg, (int)sizeof(T)) > gemm_ws_bytes) cfg.splitk = 1;
+        int e;
+        if constexpr (sizeof(T) == 8)
+            e = launch_dgemm(false, false, m, n, k, 1.0, pa + p * mk, m, pb + p * kn, k, 0.0, pt + p * mn, m, cfg,
+                             gemm_ws, s);
+        else
+            e = launch_sgemm(false, false, m, n, k, 1.0f, pa + p * mk, m, pb + p * kn, k, 0.0f, pt + p * mn, m, cfg,
+                             gemm_ws, s);
+        if (e) return e;
+    }
+    hipLaunchKernelGGL(combine3m_kernel<T>, dim3(grid_of((int64_t)m * n)), dim3(256), 0, s, m, n, pt, ar, ai, br, bi,
+                       C, ldc);
+    return (int)hipGetLastError();
+}
+
+int launch_zgemm3m(char ta, char tb, int m, int n, int k, const double *alpha, const double *A, int64_t lda,
+                   const double *B, int64_t ldb, const double *beta, double *C, int64_t ldc, void *ws, void *gemm_ws,
+                   size_t gemm_ws_bytes, int cus, hipStream_t s) {
+    return gemm3m_t<double>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ws, gemm_ws, gemm_ws_bytes, cus, s);
+}
+
+int launch_cgemm3m(char ta, char tb, int m, int n, int k, const float *alpha, const float *A, int64_t lda,
+                   const float *B, int64_t ldb, const float *beta, float *C, int64_t ldc, void *ws, void *gemm_ws,
+                   size_t gemm_ws_bytes, int cus, hipStream_t s) {
+    return gemm3m_t<float>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ws, gemm_ws, gemm_ws_bytes, cus, s);
+}
+
+GemmConfig pick_complex_gemm_config(int dt, int m, int n, int k, int cus) {
+    GemmConfig c;
+    const long t128 = (long)((m + 127) / 128) * ((n + 127) / 128);
+    const long t64 = (long)((m + 63) / 64) * ((n + 63) / 64);
+    (void)dt;
+    if (t128 >= cus) {
+        c.tile = TILE_128x128;
+    } else {
+        c.tile = TILE_64x64;
+        if (t64 < cus && k >= 512) {
+            int want = (int)((2L * cus + t64 - 1) / t64);
+            int maxk = k / 256;
+            if (want > maxk) want = maxk;
+            if (want > 8) want = 8;
+            c.splitk = want > 1 ? want : 1;
+        }
+    }
+    return c;
+}
+
+}  // namespace gemm_amd

@@ -74,18 +74,19 @@ static DevBuf g_bufs[BUF_COUNT];
 DevBuf &buf(BufId id) { return g_bufs[id]; }
 
 static std::mutex g_ws_mu;
-static DevBuf g_ws;
-void *device_workspace(size_t bytes, int *status) {
+static DevBuf g_ws[WS_COUNT];
+void *device_workspace(int slot, size_t bytes, int *status) {
     std::lock_guard<std::mutex> lk(g_ws_mu);
     int st = 0;
-    if (bytes > g_ws.bytes) {
+    DevBuf &w = g_ws[slot];
+    if (bytes > w.bytes || !w.ptr) {
         // growth must not free memory another stream may still be using
         hipError_t e = hipDeviceSynchronize();
         if (e != hipSuccess) st = hip_status(e, "hipDeviceSynchronize (workspace growth)");
-        if (st == 0) st = g_ws.ensure(bytes);
+        if (st == 0) st = w.ensure(bytes);
     }
     if (status) *status = st;
-    return st == 0 ? g_ws.ptr : nullptr;
+    return st == 0 ? w.ptr : nullptr;
 }
 
 void xerbla(const char *name, int info) {

@@ -35,9 +35,13 @@ struct DevBuf {
 // Named scratch buffers: host-pointer staging (A, B, C) and kernel workspaces.
 enum BufId { BUF_A = 0, BUF_B, BUF_C, BUF_WORK, BUF_WORK2, BUF_WORK3, BUF_COUNT };
 DevBuf &buf(BufId id);                     // for host-pointer calls (under rt().mu)
-// Workspace for device-pointer calls; guarded by its own mutex so concurrent
-// device calls on different streams serialise only on growth.
-void *device_workspace(size_t bytes, int *status);
+// Workspaces for device-pointer calls, one per slot so that a routine can
+// hold several at once (e.g. DTRSM's inverse blocks while its trailing GEMMs
+// use the split-K slot).  Growth synchronises the device first, so memory a
+// queued kernel may still use is never freed under it.
+enum WsSlot { WS_GEMM = 0, WS_TRSM_INV, WS_TRSM_TMP, WS_EMU_A, WS_EMU_B, WS_EMU_AUX, WS_COMPLEX, WS_GROUPED,
+              WS_COUNT };
+void *device_workspace(int slot, size_t bytes, int *status);
 
 // netlib xerbla message: " ** On entry to NAME parameter number I had an illegal value"
 void xerbla(const char *name, int info);

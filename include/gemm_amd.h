@@ -42,7 +42,49 @@ void dgemm_(const char *transa, const char *transb, const int *m, const int *n, 
             const double *alpha, const double *a, const int *lda, const double *b, const int *ldb,
             const double *beta, double *c, const int *ldc);
 
+/* B <- alpha * op(A)^-1 * B (side 'L') or alpha * B * op(A)^-1 (side 'R'),
+ * netlib dtrsm_ contract.  The reference ships only the kernel-level solvers
+ * for this path (dtrsm_solve_*: OpenBLAS-like_implementation/dtrsm_16x4/
+ * dtrsm_kernel_16x4_skylakex.S:1882-2536, driven by OpenBLAS's trsm driver);
+ * this is the BLAS-level routine those kernels serve.  Only the uplo triangle
+ * of A is read; diag 'U' never reads the diagonal. */
+void dtrsm_(const char *side, const char *uplo, const char *transa, const char *diag, const int *m, const int *n,
+            const double *alpha, const double *a, const int *lda, double *b, const int *ldb);
+
+/* Complex C <- alpha*op(A)*op(B) + beta*C by the 3M method (three real
+ * products), op in {N, T, C}; alpha and beta are (re, im) pairs.  Argument
+ * list of the OpenBLAS extensions cgemm3m_/zgemm3m_ (= that of cgemm_/zgemm_).
+ * The reference ships the 3M real-product kernels for this path
+ * (cgemm3m_kernel_16x4_skylakex.c:416, zgemm3m_kernel_2x8_skylakex.c:337,
+ * zgemm3m_kernel_8x4_skylakex.c:310) whose driver lives in OpenBLAS. */
+void cgemm3m_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+              const float *alpha, const float *a, const int *lda, const float *b, const int *ldb,
+              const float *beta, float *c, const int *ldc);
+void zgemm3m_(const char *transa, const char *transb, const int *m, const int *n, const int *k,
+              const double *alpha, const double *a, const int *lda, const double *b, const int *ldb,
+              const double *beta, double *c, const int *ldc);
+
 /* ===================================================== Part 2: device pointers */
+
+int gemm_amd_cgemm3m(char transa, char transb, int m, int n, int k, const float *alpha,
+                     const float *a, int lda, const float *b, int ldb, const float *beta,
+                     float *c, int ldc, void *stream);
+int gemm_amd_zgemm3m(char transa, char transb, int m, int n, int k, const double *alpha,
+                     const double *a, int lda, const double *b, int ldb, const double *beta,
+                     double *c, int ldc, void *stream);
+
+int gemm_amd_dtrsm(char side, char uplo, char transa, char diag, int m, int n, double alpha,
+                   const double *a, int lda, double *b, int ldb, void *stream);
+
+/* Kernel-level solve with the reference's packed-operand contract, on device
+ * pointers: replaces dtrsm_solve_{ln,lt,rn,rt}_m{M}n{N}(sa, sb, C, LDB, LDC)
+ * (dtrsm_kernel_16x4_skylakex.S:1882-2536; semantics as its tester's
+ * solve_*_reference, dtrsm_kernel_16x4_tester.c:97-199), for any M, N.
+ * variant 0 = ln (backward), 1 = lt (forward), 2 = rn, 3 = rt.  The triangle's
+ * diagonal holds inverse pivots; the solution is written to C and back into
+ * sb (L) or sa (R). */
+int gemm_amd_dtrsm_kernel(int variant, long M, long N, double *sa, double *sb, double *c, long LDB, long LDC,
+                          void *stream);
 
 int gemm_amd_sgemm(char transa, char transb, int m, int n, int k, float alpha,
                    const float *a, int lda, const float *b, int ldb, float beta,
