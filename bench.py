@@ -98,6 +98,9 @@ def main():
     ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-size", type=int, default=None, help="CPU leg problem size (default: --size)")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB first (a known-size kernel that "
+                         "calibrates the HBM byte counters when the run is profiled)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the DGEMM on a single MI355X "
@@ -105,6 +108,14 @@ def main():
         sys.exit(2)
 
     size = args.size
+    if args.copy_mb > 0:
+        from gemm_avx512f_amd.device import DeviceBuffer, synchronize
+        from gemm_avx512f_amd import _lib
+        nb = args.copy_mb << 20
+        src, dst = DeviceBuffer(nb), DeviceBuffer(nb)
+        _lib.check(_lib.lib().gemm_amd_memcpy(dst.ptr, src.ptr, nb, 3), "calibration copy")
+        synchronize()
+        del src, dst
     ms, wall = gpu_leg(size, args.steps, args.warmup)
     flops = 2.0 * size ** 3
     ms_step = ms / args.steps

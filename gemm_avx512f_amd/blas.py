@@ -146,3 +146,12 @@ def cgemm3m(transa, transb, m, n, k, alpha, a: DeviceMatrix, b: DeviceMatrix, be
     rc = _lib.lib().gemm_amd_cgemm3m(_enc(transa), _enc(transb), m, n, k, _cpair(alpha, ctypes.c_float), a.ptr, a.ld,
                                      b.ptr, b.ld, _cpair(beta, ctypes.c_float), c.ptr, c.ld, _stream(stream))
     return _lib.check(rc, "cgemm3m")
+
+
+# ------------------------------------------------------------------ emulated precision
+def dgemm_ozaki(transa, transb, m, n, k, alpha, a: DeviceMatrix, b: DeviceMatrix, beta, c: DeviceMatrix, nslices=7,
+                stream=None):
+    """DGEMM emulated on int8 matrix cores with `nslices` 7-bit slices per operand (Ozaki scheme)."""
+    rc = _lib.lib().gemm_amd_dgemm_ozaki(_enc(transa), _enc(transb), m, n, k, alpha, a.ptr, a.ld, b.ptr, b.ld, beta,
+                                         c.ptr, c.ld, nslices, _stream(stream))
+    return _lib.check(rc, "dgemm_ozaki")

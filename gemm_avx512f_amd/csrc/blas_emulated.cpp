@@ -1,0 +1,54 @@
+// C-ABI entry points of the emulated-precision products: the int8-slice
+// (Ozaki) DGEMM and the bf16-split SGEMM.
+#include "runtime.hpp"
+#include "ozaki.hpp"
+#include "gemm_kernels.hpp"
+#include "../../include/gemm_amd.h"
+
+using namespace gemm_amd;
+
+extern "C" int gemm_amd_dgemm_ozaki(char ta, char tb, int m, int n, int k, double alpha, const double *a, int lda,
+                                    const double *b, int ldb, double beta, double *c, int ldc, int nslices,
+                                    void *stream) {
+    const int info = gemm_check(ta, tb, m, n, k, lda, ldb, ldc);
+    if (info) {
+        xerbla("DGEMM_OZAKI", info);
+        return info;
+    }
+    if (nslices < 1 || nslices > 8) {
+        set_error("gemm_amd_dgemm_ozaki: nslices must be 1..8");
+        return -(int)hipErrorInvalidValue;
+    }
+    if (m == 0 || n == 0 || ((alpha == 0.0 || k == 0) && beta == 1.0)) return 0;
+    if (int st = init_status()) return st;
+    hipStream_t s = (hipStream_t)stream;
+    if (alpha == 0.0 || k == 0) return hip_status((hipError_t)launch_scale_d(m, n, beta, c, ldc, s), "DGEMM_OZAKI");
+    int st = 0;
+    void *wa = device_workspace(WS_EMU_A, ozaki_workspace_bytes(m, k, nslices), &st);
+    if (st) return st;
+    void *wb = device_workspace(WS_EMU_B, ozaki_workspace_bytes(n, k, nslices), &st);
+    if (st) return st;
+    void *ww = device_workspace(WS_EMU_W, ozaki_accum_bytes(m, n), &st);
+    if (st) return st;
+    const int e = launch_dgemm_ozaki(!notrans(ta), !notrans(tb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nslices,
+                                     wa, wb, ww, s);
+    return hip_status((hipError_t)e, "DGEMM_OZAKI");
+}
+
+extern "C" int gemm_amd_ozaki_split(const double *x, int64_t row_stride, int64_t k_stride, int rows, int k,
+                                    int nslices, int8_t *slices, int rows_pad, int k_pad, int *exps, void *stream) {
+    if (rows < 0 || k < 0 || nslices < 1 || nslices > 8 || rows_pad < rows || k_pad < k || rows_pad % 64 != 0 ||
+        k_pad % 64 != 0) {
+        set_error("gemm_amd_ozaki_split: need rows, k >= 0, nslices in 1..8, rows_pad >= rows and k_pad >= k, "
+                  "both multiples of 64");
+        return -(int)hipErrorInvalidValue;
+    }
+    if (rows == 0 || k == 0) return 0;
+    if (int st = init_status()) return st;
+    int st = 0;
+    void *rm = device_workspace(WS_EMU_AUX, (size_t)rows_pad * 8, &st);
+    if (st) return st;
+    return hip_status((hipError_t)launch_ozaki_split_only(x, row_stride, k_stride, rows, k, nslices, slices, rows_pad,
+                                                          k_pad, exps, (unsigned long long *)rm, (hipStream_t)stream),
+                      "ozaki_split");
+}

@@ -62,6 +62,17 @@ struct TileLoader {
         }
     }
 
+    // Interior slab: every element in range and 16-byte aligned.
+    __device__ __forceinline__ void load_full(const T *__restrict__ base, int64_t ld, int tid) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            int r, k;
+            coords(tid + i * NT, r, k);
+            const T *src = KCONTIG ? base + (int64_t)r * ld + k : base + r + (int64_t)k * ld;
+            reg[i] = *reinterpret_cast<const V *>(src);
+        }
+    }
+
     __device__ __forceinline__ void load(const T *__restrict__ base, int64_t ld, int rlim, int klim,
                                          int tid) {
 #pragma unroll
@@ -103,25 +114,29 @@ struct TileLoader {
 // FP64: v_mfma_f64_4x4x4_4b.  A 16x16 sub-tile is four instructions: all use
 // the same A fragment (rows 4b..4b+3 of the slab in block b) and rotation r
 // feeds block b the B columns 4((b+r)%4)..+3, so the four rotations cover the
-// 16 column groups x 4 row groups once.  Lane group g = lane/16 owns
-// k = 4g..4g+3 of each 16-deep LDS slab (any k assignment is valid as long as
-// A and B use the same one).
+// 16 column groups x 4 row groups once.  Each rotation's B fragment is read
+// straight from LDS (the row offset of lane q is its column 4((b+r)%4)+j):
+// VALU work between the matrix instructions (e.g. DPP lane rotations)
+// measurably starves the FP64 matrix pipe on gfx950, LDS reads do not.  Lane
+// group g = lane/16 owns k = g*BK/4 .. (g+1)*BK/4-1 of each LDS slab, read two
+// at a time (any k assignment is valid as long as A and B use the same one).
 template <int WTM, int WTN, int BK>
 struct WaveF64 {
     static constexpr int SM = WTM / 16, SN = WTN / 16;
     static constexpr int PITCH = BK + 2;
-    static_assert(BK == 16, "FP64 wave tile expects a 16-deep slab");
+    static_assert(BK % 8 == 0, "FP64 slab depth must be a multiple of 8");
     double acc[SM][SN][4];
     int aoff[SM], boff[SN][4];
 
     __device__ __forceinline__ void init(int lane, int wm0, int wn0) {
         const int q = lane & 15, g = lane >> 4, b = q >> 2, j = q & 3;
 #pragma unroll
-        for (int sm = 0; sm < SM; ++sm) aoff[sm] = (wm0 + sm * 16 + q) * PITCH + 4 * g;
+        for (int sm = 0; sm < SM; ++sm) aoff[sm] = (wm0 + sm * 16 + q) * PITCH + (BK / 4) * g;
 #pragma unroll
         for (int sn = 0; sn < SN; ++sn)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) boff[sn][r] = (wn0 + sn * 16 + 4 * ((b + r) & 3) + j) * PITCH + 4 * g;
+            for (int r = 0; r < 4; ++r)
+                boff[sn][r] = (wn0 + sn * 16 + 4 * ((b + r) & 3) + j) * PITCH + (BK / 4) * g;
 #pragma unroll
         for (int sm = 0; sm < SM; ++sm)
 #pragma unroll
@@ -132,7 +147,7 @@ struct WaveF64 {
 
     __device__ __forceinline__ void compute(const double *As, const double *Bs) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < BK / 8; ++h) {
             d2_t av[SM], bv[SN][4];
 #pragma unroll
             for (int sm = 0; sm < SM; ++sm) av[sm] = *reinterpret_cast<const d2_t *>(As + aoff[sm] + 2 * h);
@@ -149,6 +164,9 @@ struct WaveF64 {
 #pragma unroll
                         for (int r = 0; r < 4; ++r)
                             acc[sm][sn][r] = mfma_f64_4x4x4(av[sm][e], bv[sn][r][e], acc[sm][sn][r]);
+            // keep each phase's fragment reads next to its matrix ops: hoisting
+            // every phase's reads to the top would need 4x the fragment registers
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 
@@ -205,6 +223,7 @@ struct WaveF32 {
 #pragma unroll
                     for (int sn = 0; sn < SN; ++sn)
                         acc[sm][sn] = mfma_f32_32x32x2(av[sm][e], bv[sn][e], acc[sm][sn]);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 
@@ -227,7 +246,10 @@ template <int WTM, int WTN, int BK> struct WaveSel<float, WTM, WTN, BK> { using 
 
 // ---------------------------------------------------------------- kernel
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, bool VEC>
-__global__ __launch_bounds__(WM *WN * 64) void gemm_mfma_kernel(GemmParams<T> p) {
+// FP64 wave tiles of up to 64x32 fit two waves per SIMD without spilling;
+// 64x64 ones need the whole register file of one.
+__global__ __launch_bounds__(WM *WN * 64, (sizeof(T) == 8 && (BM / WM) * (BN / WN) <= 2048) ? 2 : 1)
+void gemm_mfma_kernel(GemmParams<T> p) {
     constexpr int NT = WM * WN * 64;
     constexpr int WTM = BM / WM, WTN = BN / WN;
     using LA = TileLoader<T, BM, BK, NT, TA, VEC>;    // op(A) rows = M; k-contiguous iff A is transposed
@@ -260,28 +282,54 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_mfma_kernel(GemmParams<T> p)
     w.init(lane, wm0, wn0);
 
     const int nk = (kend - kbeg + BK - 1) / BK;
+    // Interior tiles (the bulk of any large problem) stream whole aligned
+    // slabs with no per-element range checks.
+    const bool interior = VEC && mlim >= BM && nlim >= BN && (kend - kbeg) % BK == 0;
     if (nk > 0) {
-        la.load(abase, p.lda, mlim, kend - kbeg, tid);
-        lb.load(bbase, p.ldb, nlim, kend - kbeg, tid);
+        if (interior) {
+            la.load_full(abase, p.lda, tid);
+            lb.load_full(bbase, p.ldb, tid);
+        } else {
+            la.load(abase, p.lda, mlim, kend - kbeg, tid);
+            lb.load(bbase, p.ldb, nlim, kend - kbeg, tid);
+        }
         la.store(smem, tid);
         lb.store(smem + BM * PITCH, tid);
     }
     __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-        T *cur = smem + (kt & 1) * STAGE;
-        T *nxt = smem + ((kt + 1) & 1) * STAGE;
-        const bool more = kt + 1 < nk;
-        if (more) {
-            const int krem = kend - kbeg - (kt + 1) * BK;
-            la.load(abase + (kt + 1) * astep, p.lda, mlim, krem, tid);
-            lb.load(bbase + (kt + 1) * bstep, p.ldb, nlim, krem, tid);
+    if (interior) {
+        for (int kt = 0; kt < nk; ++kt) {
+            T *cur = smem + (kt & 1) * STAGE;
+            T *nxt = smem + ((kt + 1) & 1) * STAGE;
+            const bool more = kt + 1 < nk;
+            if (more) {
+                la.load_full(abase + (kt + 1) * astep, p.lda, tid);
+                lb.load_full(bbase + (kt + 1) * bstep, p.ldb, tid);
+            }
+            w.compute(cur, cur + BM * PITCH);
+            if (more) {
+                la.store(nxt, tid);
+                lb.store(nxt + BM * PITCH, tid);
+            }
+            __syncthreads();
         }
-        w.compute(cur, cur + BM * PITCH);
-        if (more) {
-            la.store(nxt, tid);
-            lb.store(nxt + BM * PITCH, tid);
+    } else {
+        for (int kt = 0; kt < nk; ++kt) {
+            T *cur = smem + (kt & 1) * STAGE;
+            T *nxt = smem + ((kt + 1) & 1) * STAGE;
+            const bool more = kt + 1 < nk;
+            if (more) {
+                const int krem = kend - kbeg - (kt + 1) * BK;
+                la.load(abase + (kt + 1) * astep, p.lda, mlim, krem, tid);
+                lb.load(bbase + (kt + 1) * bstep, p.ldb, nlim, krem, tid);
+            }
+            w.compute(cur, cur + BM * PITCH);
+            if (more) {
+                la.store(nxt, tid);
+                lb.store(nxt + BM * PITCH, tid);
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
 
     if (p.work) {
@@ -334,29 +382,30 @@ __global__ __launch_bounds__(256) void scale_kernel(int M, int N, T beta, T *__r
 }
 
 // ---------------------------------------------------------------- host launch
-struct TileShape { int BM, BN, WM, WN; };
+struct TileShape { int BM, BN, WM, WN, BK; };
 // FP64 wave tiles hold 64 FP64 accumulators (128 VGPRs) at 64x64, so FP64
 // keeps one wave per SIMD (4-wave workgroups); FP32 affords 8-wave groups.
+// The 128x128 FP64 tile (one workgroup per CU) takes 32-deep slabs to halve
+// the barriers per flop; smaller tiles keep 16 so two workgroups fit in LDS.
 static constexpr TileShape kTilesF64[TILE_COUNT] = {
-    {128, 128, 2, 2}, {64, 64, 2, 2}, {128, 64, 2, 2}, {256, 64, 4, 1}};
+    {128, 128, 2, 2, 32}, {64, 64, 2, 2, 16}, {128, 64, 2, 2, 16}, {256, 64, 4, 1, 16}};
 static constexpr TileShape kTilesF32[TILE_COUNT] = {
-    {128, 128, 2, 2}, {64, 64, 2, 2}, {128, 64, 2, 2}, {256, 128, 4, 2}};
+    {128, 128, 2, 2, 32}, {64, 64, 2, 2, 32}, {128, 64, 2, 2, 32}, {256, 128, 4, 2, 32}};
 
 static const TileShape &tile_shape(int t, int elem_bytes) {
     return elem_bytes == 8 ? kTilesF64[t] : kTilesF32[t];
 }
 int tile_bm(int t, int elem_bytes) { return tile_shape(t, elem_bytes).BM; }
 int tile_bn(int t, int elem_bytes) { return tile_shape(t, elem_bytes).BN; }
-int tile_bk(int, int elem_bytes) { return elem_bytes == 8 ? 16 : 32; }
+int tile_bk(int t, int elem_bytes) { return tile_shape(t, elem_bytes).BK; }
 
 size_t gemm_workspace_bytes(int m, int n, int, const GemmConfig &cfg, int elem_bytes) {
     if (cfg.splitk <= 1) return 0;
     return (size_t)cfg.splitk * (size_t)m * (size_t)n * (size_t)elem_bytes;
 }
 
-template <typename T, bool TA, bool TB, int BM, int BN, int WM, int WN, bool VEC>
+template <typename T, bool TA, bool TB, int BM, int BN, int WM, int WN, int BK, bool VEC>
 static hipError_t launch_one(const GemmParams<T> &p, int splits, hipStream_t s) {
-    constexpr int BK = sizeof(T) == 8 ? 16 : 32;
     constexpr int PITCH = BK + 16 / (int)sizeof(T);
     constexpr size_t lds = 2ull * (BM + BN) * PITCH * sizeof(T);
     auto kern = gemm_mfma_kernel<T, TA, TB, BM, BN, BK, WM, WN, VEC>;
@@ -375,12 +424,12 @@ static hipError_t launch_one(const GemmParams<T> &p, int splits, hipStream_t s) 
 template <typename T, bool TA, bool TB, bool VEC>
 static hipError_t launch_tile(int tile, const GemmParams<T> &p, int splits, hipStream_t s) {
     switch (tile) {
-    case TILE_128x128: return launch_one<T, TA, TB, 128, 128, 2, 2, VEC>(p, splits, s);
-    case TILE_64x64: return launch_one<T, TA, TB, 64, 64, 2, 2, VEC>(p, splits, s);
-    case TILE_128x64: return launch_one<T, TA, TB, 128, 64, 2, 2, VEC>(p, splits, s);
+    case TILE_128x128: return launch_one<T, TA, TB, 128, 128, 2, 2, 32, VEC>(p, splits, s);
+    case TILE_64x64: return launch_one<T, TA, TB, 64, 64, 2, 2, sizeof(T) == 8 ? 16 : 32, VEC>(p, splits, s);
+    case TILE_128x64: return launch_one<T, TA, TB, 128, 64, 2, 2, sizeof(T) == 8 ? 16 : 32, VEC>(p, splits, s);
     case TILE_LARGE:
-        if constexpr (sizeof(T) == 8) return launch_one<T, TA, TB, 256, 64, 4, 1, VEC>(p, splits, s);
-        else return launch_one<T, TA, TB, 256, 128, 4, 2, VEC>(p, splits, s);
+        if constexpr (sizeof(T) == 8) return launch_one<T, TA, TB, 256, 64, 4, 1, 16, VEC>(p, splits, s);
+        else return launch_one<T, TA, TB, 256, 128, 4, 2, 32, VEC>(p, splits, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -399,7 +448,7 @@ static hipError_t launch_gemm_t(bool ta, bool tb, int m, int n, int k, T alpha, 
         return hipGetLastError();
     }
     const int tile = (cfg.tile >= 0 && cfg.tile < TILE_COUNT) ? cfg.tile : TILE_128x128;
-    constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+    constexpr int BK = 32;   // split-K partitions start on a multiple of every tile's slab depth
     constexpr int VW = 16 / (int)sizeof(T);
     GemmParams<T> p;
     p.M = m; p.N = n; p.K = k;

@@ -1,0 +1,408 @@
+// Emulated DGEMM on the int8 matrix cores (Ozaki scheme, int8 slices).
+//
+// No reference counterpart: the reference computes FP64 GEMM with FP64 FMAs
+// (src/kernel/gemm_kernel.S:11-23).  Method: error-free splitting of Ozaki,
+// Ogita and Oishi (2012) with int8 slices and exact int32 accumulation as in
+// Ootomo, Ozaki and Yokota, "DGEMM on integer matrix multiplication unit"
+// (IJHPCA 2024).
+//
+//   split:  every row i of op(A) is scaled by 2^-e_i (e_i from frexp of the
+//           row's max |a|, so |a|*2^-e_i < 1) and cut into s slices with
+//           round-to-nearest: y = a*2^-e_i; v = 64*y for t = 1 and v = 128*y
+//           after, q_t = rint(v) in [-64, 64], y = v - q_t in [-1/2, 1/2] (all
+//           exact), giving a = 2^e_i * sum_t 2^(1-7t) q_t + O(2^(e_i-7s)).
+//           Rounding (not truncation) keeps the slicing error unbiased, so it
+//           grows like sqrt(K) in a dot product instead of like K.
+//           Columns of op(B) likewise (exponents f_j, slices r_u).
+//   product: C_ij = alpha * 2^(e_i+f_j) * sum_{t+u<=s+1} 2^(2-7(t+u)) (Q_t R_u)_ij
+//           + beta*C_ij.  Each Q_t R_u is an int8 GEMM accumulated exactly in
+//           int32 (|q|,|r| <= 64, so exact while the number of k terms since
+//           the last flush stays below 2^31/64^2); pairs with the same t+u
+//           share an int32 accumulator, which is flushed into an FP64
+//           accumulator scaled by 2^(2-7(t+u)).
+// One kernel runs all s(s+1)/2 slice products of a 256x256 output tile back to
+// back (the double-buffered LDS pipeline never drains between products) and
+// applies the exponents, alpha and beta in its epilogue; FP64 stays only in
+// the split, the flush and the epilogue.
+#include "common.hpp"
+#include "ozaki.hpp"
+#include <cstdlib>
+
+namespace gemm_amd {
+
+static constexpr int OZ_ROWPAD = 768;            // rows padded to a multiple of every tile height (lcm 256, 192)
+static constexpr int OZ_BK = 64;                  // k (bytes of int8) per LDS stage
+// 80-byte LDS rows: the 16-byte fragment reads of 16 consecutive rows land on
+// distinct 16-byte bank groups (80*q mod 256 is a permutation of 16*q).
+static constexpr int OZ_PITCH = OZ_BK + 16;
+static constexpr int OZ_MAXPAIRS = 36;            // s = 8
+static constexpr int OZ_FLUSH_K = 524160;         // 524160 * 64^2 < 2^31 - 1
+
+int ozaki_pad_rows(int x) { return (x + OZ_ROWPAD - 1) / OZ_ROWPAD * OZ_ROWPAD; }
+int ozaki_pad_k(int x) { return (x + 127) / 128 * 128; }
+
+size_t ozaki_workspace_bytes(int rows, int k, int nslices) {
+    const size_t rp = (size_t)ozaki_pad_rows(rows), kp = (size_t)ozaki_pad_k(k);
+    return (size_t)nslices * rp * kp + rp * sizeof(int) + rp * sizeof(unsigned long long) + 256;
+}
+
+size_t ozaki_accum_bytes(int m, int n) { return (size_t)ozaki_pad_rows(m) * (size_t)ozaki_pad_rows(n) * 12; }
+
+// ---------------------------------------------------------------- split
+// X is viewed as R rows x K columns: element (r, k) at X[r*rs + k*ks].
+__global__ __launch_bounds__(256) void oz_zero_u64_kernel(unsigned long long *p, int n) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = 0ull;
+}
+
+__device__ __forceinline__ void oz_load_tile(const double *__restrict__ X, int64_t rs, int64_t ks, int R, int K,
+                                             int r0, int k0, double (*T)[65]) {
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < 64 * 64; idx += 256) {
+        int rr, kk;
+        if (rs == 1) { rr = idx & 63; kk = idx >> 6; }   // rows contiguous in memory
+        else { kk = idx & 63; rr = idx >> 6; }           // k contiguous
+        const int r = r0 + rr, k = k0 + kk;
+        T[rr][kk] = (r < R && k < K) ? X[(int64_t)r * rs + (int64_t)k * ks] : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void oz_rowmax_kernel(const double *__restrict__ X, int64_t rs, int64_t ks, int R,
+                                                        int K, unsigned long long *__restrict__ rowmax) {
+    __shared__ double T[64][65];
+    const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
+    oz_load_tile(X, rs, ks, R, K, r0, k0, T);
+    __syncthreads();
+    const int tid = threadIdx.x, rr = tid >> 2, part = tid & 3;
+    double m = 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) m = fmax(m, fabs(T[rr][part * 16 + i]));
+    m = fmax(m, __shfl_xor(m, 1));
+    m = fmax(m, __shfl_xor(m, 2));
+    // non-negative doubles order like their bit patterns
+    if (part == 0 && r0 + rr < R && m > 0.0) atomicMax(&rowmax[r0 + rr], (unsigned long long)__double_as_longlong(m));
+}
+
+// grid (Kp/64, Rp/64): writes slices [t][Rp][Kp] (k contiguous; zero in the
+// padding) and exps[r] for every padded row.
+__global__ __launch_bounds__(256) void oz_slice_kernel(const double *__restrict__ X, int64_t rs, int64_t ks, int R,
+                                                       int K, int Rp, int Kp, int ns,
+                                                       const unsigned long long *__restrict__ rowmax,
+                                                       int8_t *__restrict__ out, int *__restrict__ exps) {
+    __shared__ double T[64][65];
+    const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
+    oz_load_tile(X, rs, ks, R, K, r0, k0, T);
+    __syncthreads();
+    const int tid = threadIdx.x, rr = tid >> 2, kq = (tid & 3) * 16;
+    const double amax = __longlong_as_double((long long)rowmax[r0 + rr]);
+    int e = 0;
+    if (amax > 0.0) frexp(amax, &e);
+    if (blockIdx.x == 0 && (tid & 3) == 0) exps[r0 + rr] = e;
+    double y[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) y[i] = ldexp(T[rr][kq + i], -e);
+    const int64_t plane = (int64_t)Rp * Kp;
+    int8_t *dst = out + (int64_t)(r0 + rr) * Kp + k0 + kq;
+    for (int t = 0; t < ns; ++t) {
+        i4_t w = {0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const double v = y[i] * (t == 0 ? 64.0 : 128.0);
+            const double q = rint(v);
+            y[i] = v - q;
+            w[i >> 2] |= ((int)q & 0xFF) << (8 * (i & 3));
+        }
+        *reinterpret_cast<i4_t *>(dst + t * plane) = w;
+    }
+}
+
+// ---------------------------------------------------------------- int8 GEMM
+// Slice product number pr of the triangle t+u <= s+1, ordered by d = t+u and
+// then t: diagonal d holds t = 1..d-1, so pr -> (t, u, closes-its-diagonal)
+// by a short scalar walk (kernel arguments are never indexed dynamically,
+// which would copy them to scratch memory).
+__device__ __forceinline__ void oz_pair(int pr, int &t, int &u, bool &last) {
+    int d = 2, base = 0;
+    while (pr >= base + (d - 1)) {
+        base += d - 1;
+        ++d;
+    }
+    t = pr - base + 1;
+    u = d - t;
+    last = (t == d - 1);
+}
+
+struct OzParams {
+    int M, N, Kp, npairs, ns, tiles_m, tiles_n;
+    int64_t planeA, planeB;       // bytes per slice plane
+    const int8_t *SA, *SB;
+    const int *ea, *eb;
+    double *W;                    // FP64 partial sums per tile, accumulator (fragment) order
+    int *Wi;                      // int32 staging of the accumulators, same order
+    double alpha, beta;
+    double *C;
+    int64_t ldc;
+};
+
+// One workgroup = one BM x BN output tile of WM x WN waves, each wave a
+// (32*TM) x (32*TN) block of v_mfma_i32_32x32x32_i8 tiles (16 int32
+// accumulators per tile per lane).  When a diagonal t+u is complete its int32
+// sums are scaled and added into the tile's FP64 partial sums W, stored in
+// accumulator order so every flush is a coalesced 512-byte row per 64 lanes;
+// after the last product a memory loop over W writes column-major C.
+template <int WM, int WN, int TM, int TN>
+__global__ __launch_bounds__(WM *WN * 64) void ozaki_gemm_kernel(OzParams p) {
+    constexpr int NT = WM * WN * 64;
+    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
+    constexpr int STAGE = (BM + BN) * OZ_PITCH;
+    constexpr int CH = OZ_BK / 16;                 // 16-byte chunks per row per stage
+    constexpr int PA = BM * CH / NT, PB = BN * CH / NT;
+    static_assert(BM * CH % NT == 0 && BN * CH % NT == 0, "stage must split evenly over the workgroup");
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave % WM) * (TM * 32), wn0 = (wave / WM) * (TN * 32);
+    int tm, tn;
+    tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, 8, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int nkt = p.Kp / OZ_BK;
+    const int total = p.npairs * nkt;
+    const int64_t wslot = ((int64_t)(tm * p.tiles_n + tn) * (WM * WN) + wave) * (TM * TN * 16 * 64);
+    double *W = p.W + wslot;
+    int *Wi = p.Wi + wslot;
+
+    i4_t ra[PA], rb[PB];
+    auto gload = [&](int st) {
+        const int pr = st / nkt, kt = st - pr * nkt;
+        int pt, pu;
+        bool pl;
+        oz_pair(pr, pt, pu, pl);
+        const int8_t *a = p.SA + (int64_t)(pt - 1) * p.planeA + (int64_t)m0 * p.Kp + kt * OZ_BK;
+        const int8_t *b = p.SB + (int64_t)(pu - 1) * p.planeB + (int64_t)n0 * p.Kp + kt * OZ_BK;
+#pragma unroll
+        for (int i = 0; i < PA; ++i) {
+            const int v = tid + i * NT, row = v / CH, ch = (v % CH) * 16;
+            ra[i] = *reinterpret_cast<const i4_t *>(a + (int64_t)row * p.Kp + ch);
+        }
+#pragma unroll
+        for (int i = 0; i < PB; ++i) {
+            const int v = tid + i * NT, row = v / CH, ch = (v % CH) * 16;
+            rb[i] = *reinterpret_cast<const i4_t *>(b + (int64_t)row * p.Kp + ch);
+        }
+    };
+    auto sstore = [&](unsigned char *buf) {
+#pragma unroll
+        for (int i = 0; i < PA; ++i) {
+            const int v = tid + i * NT, row = v / CH, ch = (v % CH) * 16;
+            *reinterpret_cast<i4_t *>(buf + row * OZ_PITCH + ch) = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < PB; ++i) {
+            const int v = tid + i * NT, row = v / CH, ch = (v % CH) * 16;
+            *reinterpret_cast<i4_t *>(buf + (BM + row) * OZ_PITCH + ch) = rb[i];
+        }
+    };
+
+    const int q = lane & 31, g = lane >> 5;
+    const int aoff = (wm0 + q) * OZ_PITCH + 16 * g;
+    const int boff = (BM + wn0 + q) * OZ_PITCH + 16 * g;
+    i16_t acc[TM][TN];
+
+    if (total > 0) {
+        gload(0);
+        sstore(smem);
+    }
+    __syncthreads();
+    // Segments of the stage sequence: one per diagonal d = t+u (split further
+    // if its k-term count could overflow int32).  Each segment starts from zero
+    // accumulators and ends with a flush; the global-load prefetch runs on the
+    // global stage index, so the pipeline does not drain between segments.
+    constexpr int CHUNK = OZ_FLUSH_K / OZ_BK;
+    int st = 0;
+    bool first = true;
+    for (int d = 2; d <= p.ns + 1; ++d) {
+        const int dstages = (d - 1) * nkt;
+        const double scale = ldexp(1.0, 2 - 7 * d);
+        for (int c0 = 0; c0 < dstages; c0 += CHUNK) {
+            const int c1 = min(dstages, c0 + CHUNK);
+#pragma unroll
+            for (int a = 0; a < TM; ++a)
+#pragma unroll
+                for (int b = 0; b < TN; ++b)
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) acc[a][b][e] = 0;
+            for (int i = c0; i < c1; ++i, ++st) {
+                const unsigned char *cur = smem + (st & 1) * STAGE;
+                unsigned char *nxt = smem + ((st + 1) & 1) * STAGE;
+                if (st + 1 < total) gload(st + 1);
+#pragma unroll
+                for (int ks = 0; ks < OZ_BK / 32; ++ks) {
+                    i4_t av[TM], bv[TN];
+#pragma unroll
+                    for (int s = 0; s < TM; ++s)
+                        av[s] = *reinterpret_cast<const i4_t *>(cur + aoff + 32 * s * OZ_PITCH + 32 * ks);
+#pragma unroll
+                    for (int s = 0; s < TN; ++s)
+                        bv[s] = *reinterpret_cast<const i4_t *>(cur + boff + 32 * s * OZ_PITCH + 32 * ks);
+#pragma unroll
+                    for (int a = 0; a < TM; ++a)
+#pragma unroll
+                        for (int b = 0; b < TN; ++b)
+                            acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[a], bv[b], acc[a][b], 0, 0, 0);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if (st + 1 < total) sstore(nxt);
+                __syncthreads();
+            }
+            // flush: the accumulators leave through plain int32 stores (which
+            // read the accumulation registers directly; converting in place
+            // would pull them all into the VGPR file), then the wave folds its
+            // own rows into the FP64 partial sums
+            // an opaque base keeps the compiler from hoisting the 16*TM*TN store
+            // addresses out of the stage loop (they would pin hundreds of VGPRs)
+            int *wb = Wi + lane;
+            __asm__ volatile("" : "+v"(wb));
+#pragma unroll
+            for (int a = 0; a < TM; ++a)
+#pragma unroll
+                for (int b = 0; b < TN; ++b)
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) wb[((a * TN + b) * 16 + e) * 64] = acc[a][b][e];
+            for (int idx = 0; idx < TM * TN * 16; ++idx) {
+                double *w = W + idx * 64 + lane;
+                const double prev = first ? 0.0 : *w;
+                *w = fma((double)Wi[idx * 64 + lane], scale, prev);
+            }
+            first = false;
+        }
+    }
+
+    // epilogue from the FP64 partial sums (a memory loop: no register arrays
+    // are indexed here): C = alpha * 2^(e_i + f_j) * W + beta * C
+    for (int idx = 0; idx < TM * TN * 16; ++idx) {
+        const int ab = idx >> 4, e = idx & 15, a = ab / TN, b = ab % TN;
+        const int i = m0 + wm0 + 32 * a + (e >> 2) * 8 + g * 4 + (e & 3);
+        const int j = n0 + wn0 + 32 * b + q;
+        const double sum = first ? 0.0 : W[idx * 64 + lane];
+        if (i < p.M && j < p.N) {
+            const double v = p.alpha * ldexp(sum, p.ea[i] + p.eb[j]);
+            double *dst = p.C + i + (int64_t)j * p.ldc;
+            *dst = (p.beta == 0.0) ? v : v + p.beta * *dst;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- host
+static int grid1(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return (int)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+}
+
+// Splits the rows of the R x K view (element (r,k) at X[r*rs + k*ks]) into ws.
+static int ozaki_split(const double *X, int64_t rs, int64_t ks, int R, int K, int ns, void *ws, int8_t **slices,
+                       int **exps, hipStream_t s) {
+    const int Rp = ozaki_pad_rows(R), Kp = ozaki_pad_k(K);
+    unsigned char *base = static_cast<unsigned char *>(ws);
+    *slices = reinterpret_cast<int8_t *>(base);
+    size_t off = (size_t)ns * Rp * Kp;
+    off = (off + 255) & ~(size_t)255;
+    unsigned long long *rowmax = reinterpret_cast<unsigned long long *>(base + off);
+    off += (size_t)Rp * sizeof(unsigned long long);
+    *exps = reinterpret_cast<int *>(base + off);
+    hipLaunchKernelGGL(oz_zero_u64_kernel, dim3(grid1(Rp)), dim3(256), 0, s, rowmax, Rp);
+    if (K > 0 && R > 0)
+        hipLaunchKernelGGL(oz_rowmax_kernel, dim3((K + 63) / 64, (R + 63) / 64), dim3(256), 0, s, X, rs, ks, R, K,
+                           rowmax);
+    hipLaunchKernelGGL(oz_slice_kernel, dim3(Kp / 64, Rp / 64), dim3(256), 0, s, X, rs, ks, R, K, Rp, Kp, ns, rowmax,
+                       *slices, *exps);
+    return (int)hipGetLastError();
+}
+
+int ozaki_pairs(int ns, int8_t *t, int8_t *u, int8_t *last) {
+    int n = 0;
+    for (int d = 2; d <= ns + 1; ++d)
+        for (int a = 1; a <= ns; ++a) {
+            const int b = d - a;
+            if (b < 1 || b > ns) continue;
+            t[n] = (int8_t)a;
+            u[n] = (int8_t)b;
+            last[n] = 0;
+            ++n;
+        }
+    // mark the last pair of every diagonal
+    for (int i = 0; i < n; ++i) last[i] = (i + 1 == n) || (t[i + 1] + u[i + 1] != t[i] + u[i]);
+    return n;
+}
+
+template <int WM, int WN, int TM, int TN>
+static int launch_oz_variant(OzParams &p, hipStream_t s) {
+    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
+    p.tiles_m = (p.M + BM - 1) / BM;
+    p.tiles_n = (p.N + BN - 1) / BN;
+    constexpr size_t lds = 2ull * (BM + BN) * OZ_PITCH;
+    auto kern = ozaki_gemm_kernel<WM, WN, TM, TN>;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t he =
+            hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (he != hipSuccess) return (int)he;
+        attr = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(p.tiles_m * p.tiles_n), dim3(WM * WN * 64), lds, s, p);
+    return (int)hipGetLastError();
+}
+
+int launch_dgemm_ozaki(bool ta, bool tb, int m, int n, int k, double alpha, const double *A, int64_t lda,
+                       const double *B, int64_t ldb, double beta, double *C, int64_t ldc, int nslices, void *wsA,
+                       void *wsB, void *wsW, hipStream_t s) {
+    if (m <= 0 || n <= 0 || k <= 0) return 0;
+    // op(A) rows: element (i, p) at A[i + p*lda] (N) or A[p + i*lda] (T)
+    int8_t *sa, *sb;
+    int *ea, *eb;
+    int e = ozaki_split(A, ta ? lda : 1, ta ? 1 : lda, m, k, nslices, wsA, &sa, &ea, s);
+    if (e) return e;
+    // op(B) columns as rows: element (j, p) = op(B)[p][j] at B[p + j*ldb] (N) or B[j + p*ldb] (T)
+    e = ozaki_split(B, tb ? 1 : ldb, tb ? ldb : 1, n, k, nslices, wsB, &sb, &eb, s);
+    if (e) return e;
+    OzParams p;
+    p.M = m;
+    p.N = n;
+    p.Kp = ozaki_pad_k(k);
+    p.planeA = (int64_t)ozaki_pad_rows(m) * p.Kp;
+    p.planeB = (int64_t)ozaki_pad_rows(n) * p.Kp;
+    p.SA = sa;
+    p.SB = sb;
+    p.ea = ea;
+    p.eb = eb;
+    p.W = static_cast<double *>(wsW);
+    p.Wi = reinterpret_cast<int *>(static_cast<char *>(wsW) + (size_t)ozaki_pad_rows(m) * ozaki_pad_rows(n) * 8);
+    p.alpha = alpha;
+    p.beta = beta;
+    p.C = C;
+    p.ldc = ldc;
+    p.ns = nslices;
+    p.npairs = nslices * (nslices + 1) / 2;
+    static const char *var = getenv("GEMM_AMD_OZ_VARIANT");
+    const int v = var ? atoi(var) : 0;
+    if (v == 1) return launch_oz_variant<2, 2, 4, 2>(p, s);   // 256x128 tile, 4 waves of 128x64
+    if (v == 2) return launch_oz_variant<2, 2, 3, 4>(p, s);   // 192x256 tile, 4 waves of 96x128
+    return launch_oz_variant<2, 4, 4, 2>(p, s);               // 256x256 tile, 8 waves of 128x64
+}
+
+}  // namespace gemm_amd
+
+namespace gemm_amd {
+
+// Exposes the split step alone: slices (padded [t][Rp][Kp]) and exponents of
+// the rows of the R x K view of X; rowmax_ws holds Rp 64-bit words.
+int launch_ozaki_split_only(const double *X, int64_t rs, int64_t ks, int R, int K, int ns, int8_t *slices,
+                            int Rp, int Kp, int *exps, unsigned long long *rowmax_ws, hipStream_t s) {
+    // the caller's planes are Rp x Kp with Rp >= R, Kp >= K, both multiples of 64
+    if (R <= 0 || K <= 0) return 0;
+    hipLaunchKernelGGL(oz_zero_u64_kernel, dim3(grid1(Rp)), dim3(256), 0, s, rowmax_ws, Rp);
+    hipLaunchKernelGGL(oz_rowmax_kernel, dim3((K + 63) / 64, (R + 63) / 64), dim3(256), 0, s, X, rs, ks, R, K,
+                       rowmax_ws);
+    hipLaunchKernelGGL(oz_slice_kernel, dim3(Kp / 64, Rp / 64), dim3(256), 0, s, X, rs, ks, R, K, Rp, Kp, ns,
+                       rowmax_ws, slices, exps);
+    return (int)hipGetLastError();
+}
+
+}  // namespace gemm_amd

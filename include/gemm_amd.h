@@ -103,6 +103,28 @@ int gemm_amd_dgemm_cfg(char transa, char transb, int m, int n, int k, double alp
                        const double *a, int lda, const double *b, int ldb, double beta,
                        double *c, int ldc, int tile, int splitk, void *stream);
 
+/* Emulated DGEMM on the int8 matrix cores (no reference counterpart: the
+ * reference's DGEMM is FP64 FMA, src/kernel/gemm_kernel.S:11-23).  Same
+ * contract as gemm_amd_dgemm; each row of op(A) / column of op(B) is scaled by
+ * a power of two and cut into nslices (1..8) round-to-nearest int8 slices of
+ * 7 bits, the nslices*(nslices+1)/2 leading slice products run as int8 GEMMs
+ * with exact int32 accumulation, and FP64 recombines them (Ozaki-Ogita-Oishi
+ * splitting; int8 form after Ootomo-Ozaki-Yokota 2024).  Accuracy ~2^(-7*nslices)
+ * relative to the row/column maxima; nslices = 8 matches or beats the native
+ * FP64 kernel's error on uniform inputs. */
+int gemm_amd_dgemm_ozaki(char transa, char transb, int m, int n, int k, double alpha,
+                         const double *a, int lda, const double *b, int ldb, double beta,
+                         double *c, int ldc, int nslices, void *stream);
+
+/* The split step of gemm_amd_dgemm_ozaki alone (no reference counterpart;
+ * exposed so the byte-exact slicing can be checked): rows of the rows x k view
+ * of x (element (r, p) at x[r*row_stride + p*k_stride]) are cut into
+ * nslices int8 planes written to `slices` as [t][rows_pad][k_pad] (zero
+ * padding; the buffer holds nslices*rows_pad*k_pad bytes) and their exponents
+ * to exps[rows_pad].  rows_pad >= rows and k_pad >= k must be multiples of 64. */
+int gemm_amd_ozaki_split(const double *x, int64_t row_stride, int64_t k_stride, int rows, int k,
+                         int nslices, int8_t *slices, int rows_pad, int k_pad, int *exps, void *stream);
+
 /* ===================================================== Part 3: runtime */
 
 int gemm_amd_init(void);                         /* 0 or negative HIP error; idempotent */

@@ -364,9 +364,10 @@ void oracle_err_f32(int64_t n, const float *c, const double *rh, double *out) {
  * x[r*row_stride + p*k_stride], p < k):
  *   e_r = the exponent with max_p |x| in [2^(e_r-1), 2^e_r)  (frexp), 0 for an all-zero row;
  *   y   = x * 2^-e_r  (|y| < 1, exact);
- *   slice t = 1..nslices:  v = y * 128 (exact); q_t = trunc(v) in [-127,127]; y = v - q_t (exact).
+ *   slice t = 1..nslices:  v = y * (t == 1 ? 64 : 128) (exact); q_t = rint(v) in [-64,64]
+ *                          (round to nearest even); y = v - q_t in [-1/2, 1/2] (exact).
  * slices[(t-1)*rows*k + r*k + p] = q_t, exps[r] = e_r.  So
- *   x = 2^e_r * sum_t q_t * 2^(-7t) + 2^e_r * 2^(-7 nslices) * y_final. */
+ *   x = 2^e_r * sum_t q_t * 2^(1-7t) + 2^e_r * 2^(1-7 nslices) * y_final. */
 void oracle_ozaki_split(int rows, int k, const double *x, int64_t row_stride, int64_t k_stride,
                         int nslices, int8_t *slices, int *exps) {
     for (int r = 0; r < rows; ++r) {
@@ -381,8 +382,8 @@ void oracle_ozaki_split(int rows, int k, const double *x, int64_t row_stride, in
         for (int p = 0; p < k; ++p) {
             double y = ldexp(x[r * row_stride + p * k_stride], -e);
             for (int t = 0; t < nslices; ++t) {
-                double v = y * 128.0;
-                double q = trunc(v);
+                double v = y * (t == 0 ? 64.0 : 128.0);
+                double q = rint(v);
                 slices[(int64_t)t * rows * k + (int64_t)r * k + p] = (int8_t)q;
                 y = v - q;
             }

@@ -252,18 +252,25 @@ def test_oracle_zgemm_vs_numpy_and_real_special_case():
 def test_ozaki_split_reconstructs_and_is_exact_for_short_mantissas():
     s = O.Stream(9)
     x = s.f64(30, 40, r=8)
-    for ns in (4, 6, 8):
+    for ns in (2, 4, 6, 8):
         sl, ex = O.ozaki_split(x, ns, by="rows")
-        assert sl.min() >= -127 and sl.max() <= 127
-        w = 2.0 ** (-7 * np.arange(1, ns + 1))
+        assert sl.min() >= -64 and sl.max() <= 64
+        w = 2.0 ** (1 - 7 * np.arange(1, ns + 1))
         recon = np.einsum("t,trk->rk", w, sl.astype(np.float64)) * (2.0 ** ex)[:, None]
         assert np.all(np.abs(recon - x) <= 2.0 ** (ex[:, None] - 7 * ns))
         amax = np.max(np.abs(x), axis=1)
         assert np.all((amax < 2.0 ** ex) & (amax >= 2.0 ** (ex - 1)))
-    # values with <= 7*ns significant bits below the row maximum reconstruct exactly
+    # round-to-nearest slicing: the residual of a random row is unbiased (a
+    # truncating split would leave a residual with the sign of x)
+    big = s.f64(1, 20000)
+    sl, ex = O.ozaki_split(big, 3, by="rows")
+    recon = np.einsum("t,trk->rk", 2.0 ** (1 - 7 * np.arange(1, 4)), sl.astype(np.float64)) * (2.0 ** ex)[:, None]
+    res = (big - recon) * np.sign(big)
+    assert abs(res.mean()) < 0.1 * np.abs(res).mean()
+    # values with <= 6 + 7*(ns-1) significant bits below the row maximum reconstruct exactly
     y = np.asfortranarray(np.round(s.f64(8, 16) * 2 ** 20) / 2 ** 20)
     sl, ex = O.ozaki_split(y, 4, by="rows")
-    recon = np.einsum("t,trk->rk", 2.0 ** (-7 * np.arange(1, 5)), sl.astype(np.float64)) * (2.0 ** ex)[:, None]
+    recon = np.einsum("t,trk->rk", 2.0 ** (1 - 7 * np.arange(1, 5)), sl.astype(np.float64)) * (2.0 ** ex)[:, None]
     assert np.array_equal(recon, y)
     # column slicing of x equals row slicing of x^T
     slc, exc = O.ozaki_split(x, 5, by="cols")
