@@ -31,10 +31,9 @@
 namespace gemm_amd {
 
 static constexpr int OZ_ROWPAD = 768;            // rows padded to a multiple of every tile height (lcm 256, 192)
-static constexpr int OZ_BK = 64;                  // k (bytes of int8) per LDS stage
-// 80-byte LDS rows: the 16-byte fragment reads of 16 consecutive rows land on
-// distinct 16-byte bank groups (80*q mod 256 is a permutation of 16*q).
-static constexpr int OZ_PITCH = OZ_BK + 16;
+// LDS rows are KB + 16 bytes (KB = k bytes per stage, 64 or 128): the 16-byte
+// fragment reads of 16 consecutive rows land on distinct 16-byte bank groups
+// (80*q and 144*q mod 256 are permutations of 16*q).
 static constexpr int OZ_MAXPAIRS = 36;            // s = 8
 static constexpr int OZ_FLUSH_K = 524160;         // 524160 * 64^2 < 2^31 - 1
 
@@ -149,12 +148,13 @@ struct OzParams {
 // sums are scaled and added into the tile's FP64 partial sums W, stored in
 // accumulator order so every flush is a coalesced 512-byte row per 64 lanes;
 // after the last product a memory loop over W writes column-major C.
-template <int WM, int WN, int TM, int TN>
+template <int WM, int WN, int TM, int TN, int KB>
 __global__ __launch_bounds__(WM *WN * 64) void ozaki_gemm_kernel(OzParams p) {
+    constexpr int KP = KB + 16;   // LDS row pitch (bytes)
     constexpr int NT = WM * WN * 64;
     constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
-    constexpr int STAGE = (BM + BN) * OZ_PITCH;
-    constexpr int CH = OZ_BK / 16;                 // 16-byte chunks per row per stage
+    constexpr int STAGE = (BM + BN) * KP;
+    constexpr int CH = KB / 16;                 // 16-byte chunks per row per stage
     constexpr int PA = BM * CH / NT, PB = BN * CH / NT;
     static_assert(BM * CH % NT == 0 && BN * CH % NT == 0, "stage must split evenly over the workgroup");
     extern __shared__ __align__(16) unsigned char smem[];
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_gemm_kernel(OzParams p) {
     int tm, tn;
     tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, 8, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
-    const int nkt = p.Kp / OZ_BK;
+    const int nkt = p.Kp / KB;
     const int total = p.npairs * nkt;
     const int64_t wslot = ((int64_t)(tm * p.tiles_n + tn) * (WM * WN) + wave) * (TM * TN * 16 * 64);
     double *W = p.W + wslot;
@@ -175,8 +175,8 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_gemm_kernel(OzParams p) {
         int pt, pu;
         bool pl;
         oz_pair(pr, pt, pu, pl);
-        const int8_t *a = p.SA + (int64_t)(pt - 1) * p.planeA + (int64_t)m0 * p.Kp + kt * OZ_BK;
-        const int8_t *b = p.SB + (int64_t)(pu - 1) * p.planeB + (int64_t)n0 * p.Kp + kt * OZ_BK;
+        const int8_t *a = p.SA + (int64_t)(pt - 1) * p.planeA + (int64_t)m0 * p.Kp + kt * KB;
+        const int8_t *b = p.SB + (int64_t)(pu - 1) * p.planeB + (int64_t)n0 * p.Kp + kt * KB;
 #pragma unroll
         for (int i = 0; i < PA; ++i) {
             const int v = tid + i * NT, row = v / CH, ch = (v % CH) * 16;
@@ -192,18 +192,18 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_gemm_kernel(OzParams p) {
 #pragma unroll
         for (int i = 0; i < PA; ++i) {
             const int v = tid + i * NT, row = v / CH, ch = (v % CH) * 16;
-            *reinterpret_cast<i4_t *>(buf + row * OZ_PITCH + ch) = ra[i];
+            *reinterpret_cast<i4_t *>(buf + row * KP + ch) = ra[i];
         }
 #pragma unroll
         for (int i = 0; i < PB; ++i) {
             const int v = tid + i * NT, row = v / CH, ch = (v % CH) * 16;
-            *reinterpret_cast<i4_t *>(buf + (BM + row) * OZ_PITCH + ch) = rb[i];
+            *reinterpret_cast<i4_t *>(buf + (BM + row) * KP + ch) = rb[i];
         }
     };
 
     const int q = lane & 31, g = lane >> 5;
-    const int aoff = (wm0 + q) * OZ_PITCH + 16 * g;
-    const int boff = (BM + wn0 + q) * OZ_PITCH + 16 * g;
+    const int aoff = (wm0 + q) * KP + 16 * g;
+    const int boff = (BM + wn0 + q) * KP + 16 * g;
     i16_t acc[TM][TN];
 
     if (total > 0) {
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_gemm_kernel(OzParams p) {
     // if its k-term count could overflow int32).  Each segment starts from zero
     // accumulators and ends with a flush; the global-load prefetch runs on the
     // global stage index, so the pipeline does not drain between segments.
-    constexpr int CHUNK = OZ_FLUSH_K / OZ_BK;
+    constexpr int CHUNK = OZ_FLUSH_K / KB;
     int st = 0;
     bool first = true;
     for (int d = 2; d <= p.ns + 1; ++d) {
@@ -234,14 +234,14 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_gemm_kernel(OzParams p) {
                 unsigned char *nxt = smem + ((st + 1) & 1) * STAGE;
                 if (st + 1 < total) gload(st + 1);
 #pragma unroll
-                for (int ks = 0; ks < OZ_BK / 32; ++ks) {
+                for (int ks = 0; ks < KB / 32; ++ks) {
                     i4_t av[TM], bv[TN];
 #pragma unroll
                     for (int s = 0; s < TM; ++s)
-                        av[s] = *reinterpret_cast<const i4_t *>(cur + aoff + 32 * s * OZ_PITCH + 32 * ks);
+                        av[s] = *reinterpret_cast<const i4_t *>(cur + aoff + 32 * s * KP + 32 * ks);
 #pragma unroll
                     for (int s = 0; s < TN; ++s)
-                        bv[s] = *reinterpret_cast<const i4_t *>(cur + boff + 32 * s * OZ_PITCH + 32 * ks);
+                        bv[s] = *reinterpret_cast<const i4_t *>(cur + boff + 32 * s * KP + 32 * ks);
 #pragma unroll
                     for (int a = 0; a < TM; ++a)
 #pragma unroll
@@ -332,13 +332,13 @@ int ozaki_pairs(int ns, int8_t *t, int8_t *u, int8_t *last) {
     return n;
 }
 
-template <int WM, int WN, int TM, int TN>
+template <int WM, int WN, int TM, int TN, int KB>
 static int launch_oz_variant(OzParams &p, hipStream_t s) {
     constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
     p.tiles_m = (p.M + BM - 1) / BM;
     p.tiles_n = (p.N + BN - 1) / BN;
-    constexpr size_t lds = 2ull * (BM + BN) * OZ_PITCH;
-    auto kern = ozaki_gemm_kernel<WM, WN, TM, TN>;
+    constexpr size_t lds = 2ull * (BM + BN) * (KB + 16);
+    auto kern = ozaki_gemm_kernel<WM, WN, TM, TN, KB>;
     static bool attr = false;
     if (!attr) {
         hipError_t he =
@@ -382,9 +382,9 @@ int launch_dgemm_ozaki(bool ta, bool tb, int m, int n, int k, double alpha, cons
     p.npairs = nslices * (nslices + 1) / 2;
     static const char *var = getenv("GEMM_AMD_OZ_VARIANT");
     const int v = var ? atoi(var) : 0;
-    if (v == 1) return launch_oz_variant<2, 2, 4, 2>(p, s);   // 256x128 tile, 4 waves of 128x64
-    if (v == 2) return launch_oz_variant<2, 2, 3, 4>(p, s);   // 192x256 tile, 4 waves of 96x128
-    return launch_oz_variant<2, 4, 4, 2>(p, s);               // 256x256 tile, 8 waves of 128x64
+    if (v == 1) return launch_oz_variant<2, 4, 4, 2, 64>(p, s);    // 256x256 tile, 8 waves of 128x64, 64-deep
+    if (v == 2) return launch_oz_variant<2, 2, 3, 4, 128>(p, s);   // 192x256 tile, 4 waves of 96x128
+    return launch_oz_variant<2, 4, 4, 2, 128>(p, s);               // 256x256 tile, 8 waves of 128x64
 }
 
 }  // namespace gemm_amd
