@@ -24,12 +24,42 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6
+PEAK_INT8_TOPS = 5000.0
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "dgemm_8192_nn_traffic.json")
 
 
-def gpu_leg(size, steps, warmup, seed=0):
+def _time(fn, stream, steps, warmup):
+    from gemm_avx512f_amd.device import Event, synchronize
+    for _ in range(warmup):
+        fn()
+    stream.synchronize()
+    synchronize()
+    e0, e1 = Event(), Event()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    ms = e0.elapsed_ms(e1)
+    synchronize()
+    return ms, time.perf_counter() - t0
+
+
+def _sampled_error(a, b, c, rows, cols):
+    """Max and RMS relative error of C at sampled entries against long-double
+    dot products of the same (downloaded) rows of A and columns of B."""
+    ref = np.array([np.dot(a[i, :].astype(np.longdouble), b[:, j].astype(np.longdouble)) for i, j in zip(rows, cols)])
+    got = c[rows, cols].astype(np.longdouble)
+    scale = np.max(np.abs(ref))
+    d = np.abs(got - ref)
+    return float(np.max(d) / scale), float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(ref * ref)))
+
+
+def gpu_leg(size, steps, warmup, nslices=8, seed=0, emulated=True):
+    """Native FP64 DGEMM (timed region = the metric) and the int8-slice
+    emulated DGEMM on the same inputs, each with its sampled error."""
     from gemm_avx512f_amd import _lib, blas
-    from gemm_avx512f_amd.device import DeviceMatrix, Event, Stream, fill_uniform, synchronize
+    from gemm_avx512f_amd.device import DeviceMatrix, Stream, fill_uniform
 
     L = _lib.lib()
     _lib.check(L.gemm_amd_init(), "init")
@@ -41,25 +71,24 @@ def gpu_leg(size, steps, warmup, seed=0):
     d = fill_uniform(A, seed, 0, stream=stream)
     d = fill_uniform(B, seed, d, stream=stream)
     fill_uniform(C, seed, d, stream=stream)
-    for _ in range(warmup):
-        blas.dgemm("N", "N", m, n, k, 1.0, A, B, 0.0, C, stream=stream)
-    stream.synchronize()
-    synchronize()
-    e0, e1 = Event(), Event()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(steps):
-        blas.dgemm("N", "N", m, n, k, 1.0, A, B, 0.0, C, stream=stream)
-    e1.record(stream)
-    ms = e0.elapsed_ms(e1)
-    synchronize()
-    wall = time.perf_counter() - t0
-    # sanity: the result is finite (parity itself is the test suite's job)
-    probe = np.empty((1, 1), dtype=np.float64, order="F")
-    C.buf.download(probe)
-    if not np.isfinite(probe).all():
+    out = {}
+    ms, wall = _time(lambda: blas.dgemm("N", "N", m, n, k, 1.0, A, B, 0.0, C, stream=stream), stream, steps, warmup)
+    out["native"] = {"ms": ms, "wall": wall}
+    rng = np.random.default_rng(12345)
+    rows, cols = rng.integers(0, m, 256), rng.integers(0, n, 256)
+    a, b = A.to_host(), B.to_host()
+    c_nat = C.to_host()
+    if not np.isfinite(c_nat).all():
         raise RuntimeError("non-finite DGEMM result")
-    return ms, wall
+    out["native"]["err"] = _sampled_error(a, b, c_nat, rows, cols)
+    del c_nat
+    if emulated:
+        C2 = DeviceMatrix(m, n, np.float64)
+        ms2, wall2 = _time(lambda: blas.dgemm_ozaki("N", "N", m, n, k, 1.0, A, B, 0.0, C2, nslices=nslices,
+                                                    stream=stream), stream, steps, warmup)
+        out["emulated"] = {"ms": ms2, "wall": wall2, "nslices": nslices,
+                           "err": _sampled_error(a, b, C2.to_host(), rows, cols)}
+    return out
 
 
 def cpu_leg(size, reps=3):
@@ -97,6 +126,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-emulated", action="store_true", help="skip the emulated (int8-slice) DGEMM leg")
+    ap.add_argument("--nslices", type=int, default=8, help="slices per operand of the emulated DGEMM")
     ap.add_argument("--cpu-size", type=int, default=None, help="CPU leg problem size (default: --size)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB first (a known-size kernel that "
@@ -116,7 +147,8 @@ def main():
         _lib.check(_lib.lib().gemm_amd_memcpy(dst.ptr, src.ptr, nb, 3), "calibration copy")
         synchronize()
         del src, dst
-    ms, wall = gpu_leg(size, args.steps, args.warmup)
+    legs = gpu_leg(size, args.steps, args.warmup, nslices=args.nslices, emulated=not args.no_emulated)
+    ms, wall = legs["native"]["ms"], legs["native"]["wall"]
     flops = 2.0 * size ** 3
     ms_step = ms / args.steps
     tflops = flops * args.steps / (ms / 1e3) / 1e12
@@ -145,8 +177,25 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(flops / (ms_step / 1e3) / 1e12, 3),
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(flops / (ms_step / 1e3) / 1e12 / PEAK_FP64_TFLOPS, 4), "traffic": traffic},
+        "native_err": {"max_rel": legs["native"]["err"][0], "rms_rel": legs["native"]["err"][1],
+                       "sample": "256 entries vs long-double dot products"},
         "wall_s_timed_region": round(wall, 3),
     }
+    if "emulated" in legs:
+        e = legs["emulated"]
+        em_tf = flops * args.steps / (e["ms"] / 1e3) / 1e12
+        npairs = e["nslices"] * (e["nslices"] + 1) // 2
+        result["emulated"] = {
+            "path": "gemm_amd_dgemm_ozaki (int8 slices, %d slices, %d int8 products)" % (e["nslices"], npairs),
+            "tflops_fp64_equiv": round(em_tf, 3),
+            "pct_fp64_peak": round(100.0 * em_tf / PEAK_FP64_TFLOPS, 2),
+            "int8_tops": round(em_tf * npairs, 1),
+            "int8_frac_of_peak": round(em_tf * npairs / PEAK_INT8_TOPS, 4),
+            "ms_per_step": round(e["ms"] / args.steps, 4),
+            "max_rel": e["err"][0], "rms_rel": e["err"][1],
+            "fp64_level": bool(e["err"][0] <= legs["native"]["err"][0]),
+            "faster_than_native": bool(em_tf > tflops),
+        }
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_leg(args.cpu_size or size)

@@ -54,6 +54,10 @@ SIGNATURES = {
                                      c_double, c_vp, c_int, c_int, c_vp]),
     "gemm_amd_ozaki_split": (c_int, [c_vp, ctypes.c_int64, ctypes.c_int64, c_int, c_int, c_int, c_vp, c_int, c_int,
                                      c_vp, c_vp]),
+    "gemm_amd_sgemm_bf16x": (c_int, [c_char, c_char, c_int, c_int, c_int, c_float, c_vp, c_int, c_vp, c_int,
+                                     c_float, c_vp, c_int, c_int, c_vp]),
+    "gemm_amd_bf16_split": (c_int, [c_vp, ctypes.c_int64, ctypes.c_int64, c_int, c_int, c_int, c_vp, c_int, c_int,
+                                    c_vp]),
     # runtime
     "gemm_amd_init": (c_int, []),
     "gemm_amd_device_count": (c_int, []),

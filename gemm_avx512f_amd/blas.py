@@ -155,3 +155,11 @@ def dgemm_ozaki(transa, transb, m, n, k, alpha, a: DeviceMatrix, b: DeviceMatrix
     rc = _lib.lib().gemm_amd_dgemm_ozaki(_enc(transa), _enc(transb), m, n, k, alpha, a.ptr, a.ld, b.ptr, b.ld, beta,
                                          c.ptr, c.ld, nslices, _stream(stream))
     return _lib.check(rc, "dgemm_ozaki")
+
+
+def sgemm_bf16x(transa, transb, m, n, k, alpha, a: DeviceMatrix, b: DeviceMatrix, beta, c: DeviceMatrix, nprod=3,
+                stream=None):
+    """SGEMM on bf16 matrix cores from exact bf16 splits: nprod=3 (bf16x3) or 6 (bf16x6)."""
+    rc = _lib.lib().gemm_amd_sgemm_bf16x(_enc(transa), _enc(transb), m, n, k, alpha, a.ptr, a.ld, b.ptr, b.ld, beta,
+                                         c.ptr, c.ld, nprod, _stream(stream))
+    return _lib.check(rc, "sgemm_bf16x")

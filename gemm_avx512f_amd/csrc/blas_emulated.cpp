@@ -2,10 +2,52 @@
 // (Ozaki) DGEMM and the bf16-split SGEMM.
 #include "runtime.hpp"
 #include "ozaki.hpp"
+#include "bf16x.hpp"
 #include "gemm_kernels.hpp"
 #include "../../include/gemm_amd.h"
 
 using namespace gemm_amd;
+
+extern "C" int gemm_amd_sgemm_bf16x(char ta, char tb, int m, int n, int k, float alpha, const float *a, int lda,
+                                    const float *b, int ldb, float beta, float *c, int ldc, int nprod, void *stream) {
+    const int info = gemm_check(ta, tb, m, n, k, lda, ldb, ldc);
+    if (info) {
+        xerbla("SGEMM_BF16X", info);
+        return info;
+    }
+    if (nprod != 3 && nprod != 6) {
+        set_error("gemm_amd_sgemm_bf16x: nprod must be 3 (2-way split) or 6 (3-way split)");
+        return -(int)hipErrorInvalidValue;
+    }
+    if (m == 0 || n == 0 || ((alpha == 0.0f || k == 0) && beta == 1.0f)) return 0;
+    if (int st = init_status()) return st;
+    hipStream_t s = (hipStream_t)stream;
+    if (alpha == 0.0f || k == 0) return hip_status((hipError_t)launch_scale_s(m, n, beta, c, ldc, s), "SGEMM_BF16X");
+    const int nsplit = nprod == 3 ? 2 : 3;
+    int st = 0;
+    void *wa = device_workspace(WS_EMU_A, bf16x_workspace_bytes(m, k, nsplit), &st);
+    if (st) return st;
+    void *wb = device_workspace(WS_EMU_B, bf16x_workspace_bytes(n, k, nsplit), &st);
+    if (st) return st;
+    const int e = launch_sgemm_bf16x(!notrans(ta), !notrans(tb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nprod,
+                                     wa, wb, s);
+    return hip_status((hipError_t)e, "SGEMM_BF16X");
+}
+
+extern "C" int gemm_amd_bf16_split(const float *x, int64_t row_stride, int64_t k_stride, int rows, int k, int nsplit,
+                                   uint16_t *planes, int rows_pad, int k_pad, void *stream) {
+    if (rows < 0 || k < 0 || nsplit < 1 || nsplit > 3 || rows_pad < rows || k_pad < k || rows_pad % 64 != 0 ||
+        k_pad % 64 != 0) {
+        set_error("gemm_amd_bf16_split: need rows, k >= 0, nsplit in 1..3, rows_pad >= rows and k_pad >= k, "
+                  "both multiples of 64");
+        return -(int)hipErrorInvalidValue;
+    }
+    if (rows == 0 || k == 0) return 0;
+    if (int st = init_status()) return st;
+    return hip_status((hipError_t)launch_bf16_split(x, row_stride, k_stride, rows, k, nsplit, planes, rows_pad, k_pad,
+                                                    (hipStream_t)stream),
+                      "bf16_split");
+}
 
 extern "C" int gemm_amd_dgemm_ozaki(char ta, char tb, int m, int n, int k, double alpha, const double *a, int lda,
                                     const double *b, int ldb, double beta, double *c, int ldc, int nslices,

@@ -125,6 +125,24 @@ int gemm_amd_dgemm_ozaki(char transa, char transb, int m, int n, int k, double a
 int gemm_amd_ozaki_split(const double *x, int64_t row_stride, int64_t k_stride, int rows, int k,
                          int nslices, int8_t *slices, int rows_pad, int k_pad, int *exps, void *stream);
 
+/* SGEMM on the bf16 matrix cores from exact bf16 splits of the FP32 operands
+ * (no reference counterpart: the reference's SGEMM is FP32 FMA,
+ * src/kernel/gemm_kernel.S:11-23).  Same contract as gemm_amd_sgemm; nprod = 3
+ * ("bf16x3": x = x0 + x1, products x0y0 + x0y1 + x1y0, ~16 bits) or 6 ("bf16x6":
+ * 3-way split, all products of weight >= 2^-16, FP32 level), FP32
+ * accumulation (Henry, Tang, Heinecke, ARITH 2019). */
+int gemm_amd_sgemm_bf16x(char transa, char transb, int m, int n, int k, float alpha,
+                         const float *a, int lda, const float *b, int ldb, float beta,
+                         float *c, int ldc, int nprod, void *stream);
+
+/* The split step of gemm_amd_sgemm_bf16x alone (no reference counterpart;
+ * exposed so the exact splitting can be checked): rows of the rows x k view of
+ * x are cut into nsplit bf16 parts (round to nearest even, each part taken from
+ * the exact FP32 remainder), written as [part][rows_pad][k_pad] uint16 with
+ * zero padding; rows_pad >= rows and k_pad >= k must be multiples of 64. */
+int gemm_amd_bf16_split(const float *x, int64_t row_stride, int64_t k_stride, int rows, int k, int nsplit,
+                        uint16_t *planes, int rows_pad, int k_pad, void *stream);
+
 /* ===================================================== Part 3: runtime */
 
 int gemm_amd_init(void);                         /* 0 or negative HIP error; idempotent */
