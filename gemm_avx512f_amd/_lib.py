@@ -58,6 +58,15 @@ SIGNATURES = {
                                      c_float, c_vp, c_int, c_int, c_vp]),
     "gemm_amd_bf16_split": (c_int, [c_vp, ctypes.c_int64, ctypes.c_int64, c_int, c_int, c_int, c_vp, c_int, c_int,
                                     c_vp]),
+    "gemm_amd_grouped_create": (c_vp, [c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                       c_vp, c_vp, c_vp]),
+    "gemm_amd_grouped_run": (c_int, [c_vp, c_vp]),
+    "gemm_amd_grouped_capture": (c_int, [c_vp, c_vp]),
+    "gemm_amd_grouped_replay": (c_int, [c_vp, c_vp]),
+    "gemm_amd_grouped_tiles": (ctypes.c_int64, [c_vp]),
+    "gemm_amd_grouped_destroy": (c_int, [c_vp]),
+    "gemm_amd_zgemm3m_batched": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                         c_vp, c_vp, c_vp]),
     # runtime
     "gemm_amd_init": (c_int, []),
     "gemm_amd_device_count": (c_int, []),

@@ -1,0 +1,137 @@
+// C-ABI of the grouped GEMM (plans, direct launch, hipGraph capture/replay)
+// and of the batched 3M complex GEMM built on it.
+#include "runtime.hpp"
+#include "grouped.hpp"
+#include "gemm3m.hpp"
+#include "../../include/gemm_amd.h"
+#include <cstdio>
+#include <string>
+#include <vector>
+
+using namespace gemm_amd;
+
+namespace {
+
+int check_all(const char *name, int count, const char *ta, const char *tb, const int *m, const int *n, const int *k,
+              const int *lda, const int *ldb, const int *ldc) {
+    for (int p = 0; p < count; ++p) {
+        const int info = gemm_check(ta[p], tb[p], m[p], n[p], k[p], lda[p], ldb[p], ldc[p]);
+        if (info) {
+            char msg[200];
+            snprintf(msg, sizeof msg, " ** On entry to %s problem %d: parameter number %d had an illegal value", name,
+                     p, info);
+            fprintf(stderr, "%s\n", msg);
+            set_error(msg);
+            return info;
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void *gemm_amd_grouped_create(int dtype, int count, const char *transa, const char *transb, const int *m,
+                              const int *n, const int *k, const double *alpha, const void *const *a, const int *lda,
+                              const void *const *b, const int *ldb, const double *beta, void *const *c,
+                              const int *ldc) {
+    if (dtype < 0 || dtype > 2 || count < 0) {
+        set_error("gemm_amd_grouped_create: dtype must be 0 (FP32), 1 (BF16 in, FP32 out) or 2 (FP64); count >= 0");
+        return nullptr;
+    }
+    if (check_all("GROUPED_GEMM", count, transa, transb, m, n, k, lda, ldb, ldc)) return nullptr;
+    if (init_status()) return nullptr;
+    int st = 0;
+    void *plan = grouped_plan_create(dtype, count, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                                     rt().cus, &st);
+    if (!plan) hip_status((hipError_t)st, "grouped plan");
+    return plan;
+}
+
+int gemm_amd_grouped_run(void *plan, void *stream) {
+    if (!plan) return -(int)hipErrorInvalidValue;
+    return hip_status((hipError_t)grouped_plan_launch(plan, (hipStream_t)stream), "grouped launch");
+}
+
+int gemm_amd_grouped_capture(void *plan, void *stream) {
+    if (!plan || !stream) {
+        set_error("gemm_amd_grouped_capture: needs a plan and a non-null stream");
+        return -(int)hipErrorInvalidValue;
+    }
+    return hip_status((hipError_t)grouped_plan_capture(plan, (hipStream_t)stream), "grouped capture");
+}
+
+int gemm_amd_grouped_replay(void *plan, void *stream) {
+    if (!plan) return -(int)hipErrorInvalidValue;
+    return hip_status((hipError_t)grouped_plan_replay(plan, (hipStream_t)stream), "grouped replay");
+}
+
+int64_t gemm_amd_grouped_tiles(void *plan) { return plan ? grouped_plan_tiles(plan) : -1; }
+
+int gemm_amd_grouped_destroy(void *plan) {
+    if (!plan) return 0;
+    hipError_t e = hipDeviceSynchronize();   // the descriptor table may still be in use
+    if (e != hipSuccess) return hip_status(e, "grouped destroy");
+    return hip_status((hipError_t)grouped_plan_destroy(plan), "grouped destroy");
+}
+
+int gemm_amd_zgemm3m_batched(int count, const char *transa, const char *transb, const int *m, const int *n,
+                             const int *k, const double *alpha, const double *const *a, const int *lda,
+                             const double *const *b, const int *ldb, const double *beta, double *const *c,
+                             const int *ldc, void *stream) {
+    if (count < 0) return -(int)hipErrorInvalidValue;
+    if (int info = check_all("ZGEMM3M_BATCHED", count, transa, transb, m, n, k, lda, ldb, ldc)) return info;
+    if (count == 0) return 0;
+    if (int st = init_status()) return st;
+    hipStream_t s = (hipStream_t)stream;
+    // scratch: per problem 3 planes of op(A), op(B) and the three real products
+    std::vector<size_t> off(count + 1, 0);
+    for (int p = 0; p < count; ++p)
+        off[p + 1] = off[p] + 3 * ((size_t)m[p] * k[p] + (size_t)k[p] * n[p] + (size_t)m[p] * n[p]);
+    int st = 0;
+    double *ws = (double *)device_workspace(WS_COMPLEX, off[count] * sizeof(double) + 256, &st);
+    if (st) return st;
+    std::vector<char> ta3, tb3;
+    std::vector<int> m3, n3, k3, la3, lb3, lc3;
+    std::vector<double> al3, be3;
+    std::vector<const void *> A3, B3;
+    std::vector<void *> C3;
+    for (int p = 0; p < count; ++p) {
+        double *pa = ws + off[p];
+        double *pb = pa + 3 * (size_t)m[p] * k[p];
+        double *pt = pb + 3 * (size_t)k[p] * n[p];
+        int e = launch_gemm3m_split_d(transa[p], transb[p], m[p], n[p], k[p], a[p], lda[p], b[p], ldb[p], pa, pb, s);
+        if (e) return hip_status((hipError_t)e, "ZGEMM3M_BATCHED split");
+        for (int q = 0; q < 3; ++q) {
+            ta3.push_back('N');
+            tb3.push_back('N');
+            m3.push_back(m[p]);
+            n3.push_back(n[p]);
+            k3.push_back(k[p]);
+            la3.push_back(m[p] > 1 ? m[p] : 1);
+            lb3.push_back(k[p] > 1 ? k[p] : 1);
+            lc3.push_back(m[p] > 1 ? m[p] : 1);
+            al3.push_back(1.0);
+            be3.push_back(0.0);
+            A3.push_back(pa + q * (size_t)m[p] * k[p]);
+            B3.push_back(pb + q * (size_t)k[p] * n[p]);
+            C3.push_back(pt + q * (size_t)m[p] * n[p]);
+        }
+    }
+    void *plan = grouped_plan_create(GROUPED_F64, 3 * count, ta3.data(), tb3.data(), m3.data(), n3.data(), k3.data(),
+                                     al3.data(), A3.data(), la3.data(), B3.data(), lb3.data(), be3.data(), C3.data(),
+                                     lc3.data(), rt().cus, &st);
+    if (!plan) return hip_status((hipError_t)st, "ZGEMM3M_BATCHED plan");
+    int e = grouped_plan_launch(plan, s);
+    for (int p = 0; p < count && !e; ++p) {
+        const double *pt = ws + off[p] + 3 * ((size_t)m[p] * k[p] + (size_t)k[p] * n[p]);
+        e = launch_gemm3m_combine_d(m[p], n[p], pt, alpha + 2 * p, beta + 2 * p, c[p], ldc[p], s);
+    }
+    hipError_t se = hipStreamSynchronize(s);   // the plan's table must outlive the launch
+    grouped_plan_destroy(plan);
+    if (e) return hip_status((hipError_t)e, "ZGEMM3M_BATCHED");
+    return hip_status(se, "ZGEMM3M_BATCHED");
+}
+
+}  // extern "C"

@@ -14,6 +14,8 @@ typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 template <typename T> struct Vec16;  // 16-byte vector of T
 template <> struct Vec16<double> { using t = d2_t; static constexpr int N = 2; };
 template <> struct Vec16<float> { using t = f4_t; static constexpr int N = 4; };
+typedef unsigned short us8_t __attribute__((ext_vector_type(8)));
+template <> struct Vec16<uint16_t> { using t = us8_t; static constexpr int N = 8; };   // bf16 bit patterns
 
 // Lane layouts of the gfx950 matrix instructions used here (each one was
 // checked on hardware by one-hot / random-operand probes):
@@ -30,6 +32,17 @@ __device__ __forceinline__ double mfma_f64_4x4x4(double a, double b, double c) {
 }
 __device__ __forceinline__ f16_t mfma_f32_32x32x2(float a, float b, f16_t c) {
     return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+
+// The GEMM epilogue alpha*v + beta*c, spelled with one explicit fused
+// multiply-add so that every kernel rounds it identically (beta == 0 does not
+// read c).
+template <typename T>
+__device__ __forceinline__ T axpby(T alpha, T v, T beta, const T *c) {
+    return beta == T(0) ? alpha * v : __builtin_fma(beta, *c, alpha * v);
+}
+__device__ __forceinline__ float axpby(float alpha, float v, float beta, const float *c) {
+    return beta == 0.0f ? alpha * v : __builtin_fmaf(beta, *c, alpha * v);
 }
 
 // Map a linear workgroup id to a tile id so that each of the 8 XCDs (which

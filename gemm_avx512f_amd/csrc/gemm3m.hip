@@ -133,6 +133,30 @@ int launch_cgemm3m(char ta, char tb, int m, int n, int k, const float *alpha, co
     return gemm3m_t<float>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ws, gemm_ws, gemm_ws_bytes, cus, s);
 }
 
+// Pieces of the 3M method for callers that batch the three real products
+// themselves (the grouped zgemm3m batch): planes of op(A) (3 x m*k) and op(B)
+// (3 x k*n), and the final combine of the products t (3 x m*n).
+int launch_gemm3m_split_d(char ta, char tb, int m, int n, int k, const double *A, int64_t lda, const double *B,
+                          int64_t ldb, double *pa, double *pb, hipStream_t s) {
+    const bool tA = !(ta == 'N' || ta == 'n'), cA = (ta == 'C' || ta == 'c');
+    const bool tB = !(tb == 'N' || tb == 'n'), cB = (tb == 'C' || tb == 'c');
+    if ((int64_t)m * k > 0)
+        hipLaunchKernelGGL(split_planes_kernel<double>, dim3(grid_of((int64_t)m * k)), dim3(256), 0, s, m, k, A, lda,
+                           tA, cA, pa);
+    if ((int64_t)k * n > 0)
+        hipLaunchKernelGGL(split_planes_kernel<double>, dim3(grid_of((int64_t)k * n)), dim3(256), 0, s, k, n, B, ldb,
+                           tB, cB, pb);
+    return (int)hipGetLastError();
+}
+
+int launch_gemm3m_combine_d(int m, int n, const double *t, const double *alpha, const double *beta, double *C,
+                            int64_t ldc, hipStream_t s) {
+    if ((int64_t)m * n == 0) return 0;
+    hipLaunchKernelGGL(combine3m_kernel<double>, dim3(grid_of((int64_t)m * n)), dim3(256), 0, s, m, n, t, alpha[0],
+                       alpha[1], beta[0], beta[1], C, ldc);
+    return (int)hipGetLastError();
+}
+
 GemmConfig pick_complex_gemm_config(int dt, int m, int n, int k, int cus) {
     GemmConfig c;
     const long t128 = (long)((m + 127) / 128) * ((n + 127) / 128);

@@ -19,230 +19,10 @@
 //     pass reduces them in a fixed order (deterministic).
 #include "common.hpp"
 #include "gemm_kernels.hpp"
+#include "gemm_tile.hpp"
 
 namespace gemm_amd {
 
-template <typename T>
-struct GemmParams {
-    int M, N, K;
-    T alpha, beta;
-    const T *A;
-    int64_t lda;
-    const T *B;
-    int64_t ldb;
-    T *C;
-    int64_t ldc;
-    int tiles_m, tiles_n;
-    int kchunk;   // K extent per split-K partition (multiple of BK)
-    T *work;      // split-K partials, [split][N][M], or nullptr
-};
-
-// ---------------------------------------------------------------- tile loader
-// Moves a ROWS x BK slab of an operand (row = M index of op(A) or N index of
-// op(B)) into LDS laid out [row][PITCH].  KCONTIG: the operand is stored with
-// k contiguous (element (r,k) at base[k + r*ld]); otherwise rows contiguous
-// (element (r,k) at base[r + k*ld]).  Elements outside (rlim, klim) read 0.
-template <typename T, int ROWS, int BK, int NT, bool KCONTIG, bool VEC>
-struct TileLoader {
-    using V = typename Vec16<T>::t;
-    static constexpr int VW = Vec16<T>::N;
-    static constexpr int PITCH = BK + VW;
-    static constexpr int NVEC = ROWS * BK / VW;
-    static_assert(NVEC % NT == 0, "tile must split evenly over the workgroup");
-    static constexpr int PER = NVEC / NT;
-    V reg[PER];
-
-    __device__ __forceinline__ static void coords(int v, int &r, int &k) {
-        if (KCONTIG) {
-            r = v / (BK / VW);
-            k = (v % (BK / VW)) * VW;
-        } else {
-            k = v / (ROWS / VW);
-            r = (v % (ROWS / VW)) * VW;
-        }
-    }
-
-    // Interior slab: every element in range and 16-byte aligned.
-    __device__ __forceinline__ void load_full(const T *__restrict__ base, int64_t ld, int tid) {
-#pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            int r, k;
-            coords(tid + i * NT, r, k);
-            const T *src = KCONTIG ? base + (int64_t)r * ld + k : base + r + (int64_t)k * ld;
-            reg[i] = *reinterpret_cast<const V *>(src);
-        }
-    }
-
-    __device__ __forceinline__ void load(const T *__restrict__ base, int64_t ld, int rlim, int klim,
-                                         int tid) {
-#pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            int r, k;
-            coords(tid + i * NT, r, k);
-            const T *src = KCONTIG ? base + (int64_t)r * ld + k : base + r + (int64_t)k * ld;
-            const bool full = KCONTIG ? (r < rlim && k + VW <= klim) : (r + VW <= rlim && k < klim);
-            if (VEC && full) {
-                reg[i] = *reinterpret_cast<const V *>(src);
-            } else {
-                V t;
-#pragma unroll
-                for (int e = 0; e < VW; ++e) {
-                    const bool ok = KCONTIG ? (r < rlim && k + e < klim) : (r + e < rlim && k < klim);
-                    t[e] = ok ? src[e] : T(0);
-                }
-                reg[i] = t;
-            }
-        }
-    }
-
-    __device__ __forceinline__ void store(T *lds, int tid) const {
-#pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            int r, k;
-            coords(tid + i * NT, r, k);
-            if (KCONTIG) {
-                *reinterpret_cast<V *>(lds + r * PITCH + k) = reg[i];
-            } else {
-#pragma unroll
-                for (int e = 0; e < VW; ++e) lds[(r + e) * PITCH + k] = reg[i][e];
-            }
-        }
-    }
-};
-
-// ---------------------------------------------------------------- wave tiles
-// FP64: v_mfma_f64_4x4x4_4b.  A 16x16 sub-tile is four instructions: all use
-// the same A fragment (rows 4b..4b+3 of the slab in block b) and rotation r
-// feeds block b the B columns 4((b+r)%4)..+3, so the four rotations cover the
-// 16 column groups x 4 row groups once.  Each rotation's B fragment is read
-// straight from LDS (the row offset of lane q is its column 4((b+r)%4)+j):
-// VALU work between the matrix instructions (e.g. DPP lane rotations)
-// measurably starves the FP64 matrix pipe on gfx950, LDS reads do not.  Lane
-// group g = lane/16 owns k = g*BK/4 .. (g+1)*BK/4-1 of each LDS slab, read two
-// at a time (any k assignment is valid as long as A and B use the same one).
-template <int WTM, int WTN, int BK>
-struct WaveF64 {
-    static constexpr int SM = WTM / 16, SN = WTN / 16;
-    static constexpr int PITCH = BK + 2;
-    static_assert(BK % 8 == 0, "FP64 slab depth must be a multiple of 8");
-    double acc[SM][SN][4];
-    int aoff[SM], boff[SN][4];
-
-    __device__ __forceinline__ void init(int lane, int wm0, int wn0) {
-        const int q = lane & 15, g = lane >> 4, b = q >> 2, j = q & 3;
-#pragma unroll
-        for (int sm = 0; sm < SM; ++sm) aoff[sm] = (wm0 + sm * 16 + q) * PITCH + (BK / 4) * g;
-#pragma unroll
-        for (int sn = 0; sn < SN; ++sn)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                boff[sn][r] = (wn0 + sn * 16 + 4 * ((b + r) & 3) + j) * PITCH + (BK / 4) * g;
-#pragma unroll
-        for (int sm = 0; sm < SM; ++sm)
-#pragma unroll
-            for (int sn = 0; sn < SN; ++sn)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) acc[sm][sn][r] = 0.0;
-    }
-
-    __device__ __forceinline__ void compute(const double *As, const double *Bs) {
-#pragma unroll
-        for (int h = 0; h < BK / 8; ++h) {
-            d2_t av[SM], bv[SN][4];
-#pragma unroll
-            for (int sm = 0; sm < SM; ++sm) av[sm] = *reinterpret_cast<const d2_t *>(As + aoff[sm] + 2 * h);
-#pragma unroll
-            for (int sn = 0; sn < SN; ++sn)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) bv[sn][r] = *reinterpret_cast<const d2_t *>(Bs + boff[sn][r] + 2 * h);
-#pragma unroll
-            for (int e = 0; e < 2; ++e)
-#pragma unroll
-                for (int sm = 0; sm < SM; ++sm)
-#pragma unroll
-                    for (int sn = 0; sn < SN; ++sn)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            acc[sm][sn][r] = mfma_f64_4x4x4(av[sm][e], bv[sn][r][e], acc[sm][sn][r]);
-            // keep each phase's fragment reads next to its matrix ops: hoisting
-            // every phase's reads to the top would need 4x the fragment registers
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-
-    // visit(row, col, value) for every accumulator element, tile-relative
-    template <typename F>
-    __device__ __forceinline__ void for_each(int lane, int wm0, int wn0, F &&visit) const {
-        const int i = lane >> 4, b = (lane >> 2) & 3, j = lane & 3;
-#pragma unroll
-        for (int sm = 0; sm < SM; ++sm)
-#pragma unroll
-            for (int sn = 0; sn < SN; ++sn)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    visit(wm0 + sm * 16 + 4 * b + i, wn0 + sn * 16 + 4 * ((b + r) & 3) + j, acc[sm][sn][r]);
-    }
-};
-
-// FP32: v_mfma_f32_32x32x2_f32.  Lane group g = lane/32 owns k = g*BK/2 ..
-// (g+1)*BK/2 - 1 of each slab, read 4 at a time.
-template <int WTM, int WTN, int BK>
-struct WaveF32 {
-    static constexpr int SM = WTM / 32, SN = WTN / 32;
-    static constexpr int PITCH = BK + 4;
-    static_assert(BK % 8 == 0, "FP32 slab depth must be a multiple of 8");
-    f16_t acc[SM][SN];
-    int aoff[SM], boff[SN];
-
-    __device__ __forceinline__ void init(int lane, int wm0, int wn0) {
-        const int q = lane & 31, g = lane >> 5;
-#pragma unroll
-        for (int sm = 0; sm < SM; ++sm) aoff[sm] = (wm0 + sm * 32 + q) * PITCH + (BK / 2) * g;
-#pragma unroll
-        for (int sn = 0; sn < SN; ++sn) boff[sn] = (wn0 + sn * 32 + q) * PITCH + (BK / 2) * g;
-#pragma unroll
-        for (int sm = 0; sm < SM; ++sm)
-#pragma unroll
-            for (int sn = 0; sn < SN; ++sn)
-#pragma unroll
-                for (int e = 0; e < 16; ++e) acc[sm][sn][e] = 0.0f;
-    }
-
-    __device__ __forceinline__ void compute(const float *As, const float *Bs) {
-#pragma unroll
-        for (int h = 0; h < BK / 8; ++h) {
-            f4_t av[SM], bv[SN];
-#pragma unroll
-            for (int sm = 0; sm < SM; ++sm) av[sm] = *reinterpret_cast<const f4_t *>(As + aoff[sm] + 4 * h);
-#pragma unroll
-            for (int sn = 0; sn < SN; ++sn) bv[sn] = *reinterpret_cast<const f4_t *>(Bs + boff[sn] + 4 * h);
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-#pragma unroll
-                for (int sm = 0; sm < SM; ++sm)
-#pragma unroll
-                    for (int sn = 0; sn < SN; ++sn)
-                        acc[sm][sn] = mfma_f32_32x32x2(av[sm][e], bv[sn][e], acc[sm][sn]);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-
-    template <typename F>
-    __device__ __forceinline__ void for_each(int lane, int wm0, int wn0, F &&visit) const {
-        const int j = lane & 31, g = lane >> 5;
-#pragma unroll
-        for (int sm = 0; sm < SM; ++sm)
-#pragma unroll
-            for (int sn = 0; sn < SN; ++sn)
-#pragma unroll
-                for (int e = 0; e < 16; ++e)
-                    visit(wm0 + sm * 32 + (e >> 2) * 8 + g * 4 + (e & 3), wn0 + sn * 32 + j, acc[sm][sn][e]);
-    }
-};
-
-template <typename T, int WTM, int WTN, int BK> struct WaveSel;
-template <int WTM, int WTN, int BK> struct WaveSel<double, WTM, WTN, BK> { using type = WaveF64<WTM, WTN, BK>; };
-template <int WTM, int WTN, int BK> struct WaveSel<float, WTM, WTN, BK> { using type = WaveF32<WTM, WTN, BK>; };
 
 // ---------------------------------------------------------------- kernel
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, bool VEC>
@@ -348,7 +128,7 @@ void gemm_mfma_kernel(GemmParams<T> p) {
             w.for_each(lane, wm0, wn0, [&](int r, int c, T v) {
                 if (r < mlim && c < nlim) {
                     T *dst = cbase + r + (int64_t)c * p.ldc;
-                    *dst = alpha * v + beta * *dst;
+                    *dst = axpby(alpha, v, beta, dst);
                 }
             });
         }
@@ -366,7 +146,7 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(int M, int N, int sp
         const int i = (int)(idx % M);
         const int64_t j = idx / M;
         T *dst = C + i + j * ldc;
-        *dst = (beta == T(0)) ? alpha * s : alpha * s + beta * *dst;
+        *dst = axpby(alpha, s, beta, dst);
     }
 }
 

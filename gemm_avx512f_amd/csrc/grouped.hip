@@ -1,0 +1,253 @@
+// Grouped GEMM: many independent problems of different shapes and transposes
+// in ONE persistent launch (no reference counterpart: the reference's drivers
+// handle one GEMM per call, src/gemm_driver.c:116; its closest analogue is the
+// CAS-claimed tile queue of OpenBLAS-like_implementation/dgemm.c:464-482).
+//
+// The host sorts the problems by work (largest first), lays their output
+// tiles out back to back (tile_base = exclusive prefix sum) and uploads one
+// descriptor per problem.  Each workgroup of the persistent grid walks the
+// global tile sequence with a fixed stride (tile = blockIdx.x + i*gridDim.x),
+// so big tiles are spread over all CUs first and no atomic counter has to be
+// reset between launches (the launch is replayable from a captured hipGraph).
+// A tile finds its problem by binary search over tile_base and runs the same
+// LDS-staged matrix-core tile as the single-problem kernel, with per-element
+// range checks (problem sizes are arbitrary multiples of 1).
+#include "common.hpp"
+#include "gemm_tile.hpp"
+#include "grouped.hpp"
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+namespace gemm_amd {
+
+struct GroupDesc {
+    int m, n, k, ta, tb, tiles_m, tiles_n, pad;
+    int64_t lda, ldb, ldc, tile_base;
+    const void *A, *B;
+    void *C;
+    double alpha, beta;
+};
+
+// One BM x BN output tile of problem d; TI = operand element type, TO = C's.
+template <typename TI, typename TO, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN>
+__device__ void grouped_tile(const GroupDesc &d, int tm, int tn, unsigned char *smem_raw) {
+    constexpr int NT = WM * WN * 64;
+    constexpr int WTM = BM / WM, WTN = BN / WN;
+    using LA = TileLoader<TI, BM, BK, NT, TA, false>;
+    using LB = TileLoader<TI, BN, BK, NT, !TB, false>;
+    using Wave = typename WaveSel<TI, WTM, WTN, BK>::type;
+    constexpr int PITCH = LA::PITCH;
+    constexpr int STAGE = (BM + BN) * PITCH;
+    TI *smem = reinterpret_cast<TI *>(smem_raw);
+    const TI *A = static_cast<const TI *>(d.A);
+    const TI *B = static_cast<const TI *>(d.B);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave % WM) * WTM, wn0 = (wave / WM) * WTN;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int mlim = d.m - m0, nlim = d.n - n0;
+    const TI *abase = TA ? A + (int64_t)m0 * d.lda : A + m0;
+    const TI *bbase = TB ? B + n0 : B + (int64_t)n0 * d.ldb;
+    const int64_t astep = TA ? BK : (int64_t)BK * d.lda;
+    const int64_t bstep = TB ? (int64_t)BK * d.ldb : BK;
+    LA la;
+    LB lb;
+    Wave w;
+    w.init(lane, wm0, wn0);
+    const int nk = (d.k + BK - 1) / BK;
+    if (nk > 0) {
+        la.load(abase, d.lda, mlim, d.k, tid);
+        lb.load(bbase, d.ldb, nlim, d.k, tid);
+        la.store(smem, tid);
+        lb.store(smem + BM * PITCH, tid);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        TI *cur = smem + (kt & 1) * STAGE;
+        TI *nxt = smem + ((kt + 1) & 1) * STAGE;
+        const bool more = kt + 1 < nk;
+        if (more) {
+            const int krem = d.k - (kt + 1) * BK;
+            la.load(abase + (kt + 1) * astep, d.lda, mlim, krem, tid);
+            lb.load(bbase + (kt + 1) * bstep, d.ldb, nlim, krem, tid);
+        }
+        w.compute(cur, cur + BM * PITCH);
+        if (more) {
+            la.store(nxt, tid);
+            lb.store(nxt + BM * PITCH, tid);
+        }
+        __syncthreads();
+    }
+    TO *cbase = static_cast<TO *>(d.C) + m0 + (int64_t)n0 * d.ldc;
+    const TO alpha = (TO)d.alpha, beta = (TO)d.beta;
+    w.for_each(lane, wm0, wn0, [&](int r, int c, TO v) {
+        if (r < mlim && c < nlim) {
+            TO *dst = cbase + r + (int64_t)c * d.ldc;
+            *dst = axpby(alpha, v, beta, dst);
+        }
+    });
+}
+
+template <typename TI, typename TO, int BM, int BN, int BK, int WM, int WN>
+__global__ __launch_bounds__(WM *WN * 64) void grouped_gemm_kernel(const GroupDesc *__restrict__ descs, int count,
+                                                                   int64_t total_tiles) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    for (int64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+        int lo = 0, hi = count - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (descs[mid].tile_base <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const GroupDesc d = descs[lo];
+        const int local = (int)(t - d.tile_base);
+        const int tm = local % d.tiles_m, tn = local / d.tiles_m;
+        switch (d.ta * 2 + d.tb) {
+        case 0: grouped_tile<TI, TO, false, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        case 1: grouped_tile<TI, TO, false, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        case 2: grouped_tile<TI, TO, true, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        default: grouped_tile<TI, TO, true, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        }
+        __syncthreads();   // the next tile reuses the LDS stages
+    }
+}
+
+// ---------------------------------------------------------------- host side
+struct GroupedPlan {
+    int dtype = 0, count = 0;
+    int64_t total_tiles = 0;
+    GroupDesc *d_descs = nullptr;
+    int grid = 0;
+    size_t lds = 0;
+    const void *kernel = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+// Tile configuration per dtype: 64x64 tiles of 4 waves (problems run 32..2048).
+template <typename TI, typename TO>
+struct GroupedCfg;
+template <> struct GroupedCfg<float, float> { static constexpr int BM = 64, BN = 64, BK = 32, WM = 2, WN = 2; };
+template <> struct GroupedCfg<double, double> { static constexpr int BM = 64, BN = 64, BK = 16, WM = 2, WN = 2; };
+template <> struct GroupedCfg<uint16_t, float> { static constexpr int BM = 64, BN = 64, BK = 32, WM = 2, WN = 2; };
+
+template <typename TI, typename TO>
+static int prepare(GroupedPlan &P) {
+    using C = GroupedCfg<TI, TO>;
+    constexpr int PITCH = C::BK + 16 / (int)sizeof(TI);
+    P.lds = 2ull * (C::BM + C::BN) * PITCH * sizeof(TI);
+    auto kern = grouped_gemm_kernel<TI, TO, C::BM, C::BN, C::BK, C::WM, C::WN>;
+    P.kernel = reinterpret_cast<const void *>(kern);
+    hipError_t e = hipFuncSetAttribute(P.kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds);
+    return (int)e;
+}
+
+static int tile_bm_of(int dtype) { return 64; }
+static int tile_bn_of(int dtype) { return 64; }
+
+void *grouped_plan_create(int dtype, int count, const char *ta, const char *tb, const int *m, const int *n,
+                          const int *k, const double *alpha, const void *const *A, const int *lda,
+                          const void *const *B, const int *ldb, const double *beta, void *const *Cp, const int *ldc,
+                          int cus, int *status) {
+    *status = 0;
+    GroupedPlan *P = new GroupedPlan();
+    P->dtype = dtype;
+    P->count = count;
+    std::vector<int> order(count);
+    std::iota(order.begin(), order.end(), 0);
+    // largest problems first: their tiles are spread over all CUs before the small ones
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return (double)m[x] * n[x] * k[x] > (double)m[y] * n[y] * k[y];
+    });
+    std::vector<GroupDesc> h(count);
+    int64_t base = 0;
+    const int BM = tile_bm_of(dtype), BN = tile_bn_of(dtype);
+    for (int i = 0; i < count; ++i) {
+        const int p = order[i];
+        GroupDesc &d = h[i];
+        d.m = m[p];
+        d.n = n[p];
+        d.k = k[p];
+        d.ta = !(ta[p] == 'N' || ta[p] == 'n');
+        d.tb = !(tb[p] == 'N' || tb[p] == 'n');
+        d.tiles_m = (m[p] + BM - 1) / BM;
+        d.tiles_n = (n[p] + BN - 1) / BN;
+        d.pad = 0;
+        d.lda = lda[p];
+        d.ldb = ldb[p];
+        d.ldc = ldc[p];
+        d.A = A[p];
+        d.B = B[p];
+        d.C = Cp[p];
+        d.alpha = alpha[p];
+        d.beta = beta[p];
+        d.tile_base = base;
+        base += (int64_t)d.tiles_m * d.tiles_n;
+    }
+    P->total_tiles = base;
+    hipError_t e = hipMalloc(&P->d_descs, sizeof(GroupDesc) * (count > 0 ? count : 1));
+    if (e == hipSuccess && count > 0)
+        e = hipMemcpy(P->d_descs, h.data(), sizeof(GroupDesc) * count, hipMemcpyHostToDevice);
+    int st = (int)e;
+    if (!st) {
+        if (dtype == GROUPED_F32) st = prepare<float, float>(*P);
+        else if (dtype == GROUPED_F64) st = prepare<double, double>(*P);
+        else st = prepare<uint16_t, float>(*P);
+    }
+    // persistent grid: enough workgroups for every CU to hold several at once
+    int64_t g = (int64_t)cus * 4;
+    if (g > P->total_tiles) g = P->total_tiles;
+    P->grid = (int)(g > 0 ? g : 1);
+    if (st) {
+        grouped_plan_destroy(P);
+        *status = st;
+        return nullptr;
+    }
+    return P;
+}
+
+int grouped_plan_launch(void *plan, hipStream_t s) {
+    GroupedPlan *P = static_cast<GroupedPlan *>(plan);
+    if (P->total_tiles == 0) return 0;
+    void *args[] = {&P->d_descs, &P->count, &P->total_tiles};
+    return (int)hipLaunchKernel(P->kernel, dim3(P->grid), dim3(256), args, P->lds, s);
+}
+
+int grouped_plan_capture(void *plan, hipStream_t s) {
+    GroupedPlan *P = static_cast<GroupedPlan *>(plan);
+    if (P->exec) return 0;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return (int)e;
+    int le = grouped_plan_launch(plan, s);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    if (le) {
+        if (g) hipGraphDestroy(g);
+        return le;
+    }
+    if (e != hipSuccess) return (int)e;
+    P->graph = g;
+    e = hipGraphInstantiate(&P->exec, g, nullptr, nullptr, 0);
+    return (int)e;
+}
+
+int grouped_plan_replay(void *plan, hipStream_t s) {
+    GroupedPlan *P = static_cast<GroupedPlan *>(plan);
+    if (!P->exec) return (int)hipErrorInvalidValue;
+    return (int)hipGraphLaunch(P->exec, s);
+}
+
+int grouped_plan_destroy(void *plan) {
+    GroupedPlan *P = static_cast<GroupedPlan *>(plan);
+    if (!P) return 0;
+    hipError_t e = hipSuccess;
+    if (P->exec) e = hipGraphExecDestroy(P->exec);
+    if (P->graph) hipGraphDestroy(P->graph);
+    if (P->d_descs) hipFree(P->d_descs);
+    delete P;
+    return (int)e;
+}
+
+int64_t grouped_plan_tiles(void *plan) { return static_cast<GroupedPlan *>(plan)->total_tiles; }
+
+}  // namespace gemm_amd

@@ -143,6 +143,36 @@ int gemm_amd_sgemm_bf16x(char transa, char transb, int m, int n, int k, float al
 int gemm_amd_bf16_split(const float *x, int64_t row_stride, int64_t k_stride, int rows, int k, int nsplit,
                         uint16_t *planes, int rows_pad, int k_pad, void *stream);
 
+/* Grouped GEMM: `count` independent column-major problems
+ * C_p <- alpha_p*op(A_p)*op(B_p) + beta_p*C_p of arbitrary shapes and
+ * transposes, run as ONE persistent launch (no reference counterpart; the
+ * reference runs one GEMM per call, src/gemm_driver.c:116).  All pointer
+ * arrays live on the host and point to device memory.  dtype 0 = FP32,
+ * 1 = bf16 operands (uint16 bit patterns) with FP32 C and accumulation,
+ * 2 = FP64.  A plan holds the device problem table (built once); _run launches
+ * it, _capture records that launch into a hipGraph on a non-null stream and
+ * _replay relaunches the graph with no host-side work.  Problems are
+ * scheduled largest first; results do not depend on the schedule.
+ * _create returns NULL on an illegal argument (message in
+ * gemm_amd_last_error()); _destroy synchronises the device first. */
+void *gemm_amd_grouped_create(int dtype, int count, const char *transa, const char *transb, const int *m,
+                              const int *n, const int *k, const double *alpha, const void *const *a,
+                              const int *lda, const void *const *b, const int *ldb, const double *beta,
+                              void *const *c, const int *ldc);
+int gemm_amd_grouped_run(void *plan, void *stream);
+int gemm_amd_grouped_capture(void *plan, void *stream);
+int gemm_amd_grouped_replay(void *plan, void *stream);
+int64_t gemm_amd_grouped_tiles(void *plan);
+int gemm_amd_grouped_destroy(void *plan);
+
+/* A batch of independent complex-double 3M products (zgemm3m_ contract per
+ * problem; alpha/beta hold 2*count doubles): the three real products of every
+ * problem run together in one grouped FP64 launch. */
+int gemm_amd_zgemm3m_batched(int count, const char *transa, const char *transb, const int *m, const int *n,
+                             const int *k, const double *alpha, const double *const *a, const int *lda,
+                             const double *const *b, const int *ldb, const double *beta, double *const *c,
+                             const int *ldc, void *stream);
+
 /* ===================================================== Part 3: runtime */
 
 int gemm_amd_init(void);                         /* 0 or negative HIP error; idempotent */
