@@ -67,6 +67,20 @@ SIGNATURES = {
     "gemm_amd_grouped_destroy": (c_int, [c_vp]),
     "gemm_amd_zgemm3m_batched": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                          c_vp, c_vp, c_vp]),
+    # autotuner, picker, training kernels
+    "gemm_amd_tune_candidates": (c_int, [c_vp, c_vp]),
+    "gemm_amd_tune_sweep": (c_int, [c_int, c_char, c_char, c_int, c_int, c_int, c_int, c_vp]),
+    "gemm_amd_pick_config": (c_int, [c_int, c_char, c_char, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp]),
+    "gemm_amd_picker_load": (c_int, [c_vp, c_int]),
+    "gemm_amd_picker_set_mode": (c_int, [c_int]),
+    "gemm_amd_picker_choose": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "gemm_amd_tune_table_set": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "gemm_amd_tune_table_clear": (c_int, []),
+    "gemm_amd_mlp_forward": (c_int, [c_vp, c_vp, c_int, c_vp, c_vp]),
+    "gemm_amd_mlp_grad": (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp]),
+    "gemm_amd_mlp_adam": (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, c_float, c_float, c_float, c_float, c_int, c_vp]),
+    "gemm_amd_mlp_train": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_int, c_float,
+                                   ctypes.c_uint64, c_vp]),
     # runtime
     "gemm_amd_init": (c_int, []),
     "gemm_amd_device_count": (c_int, []),

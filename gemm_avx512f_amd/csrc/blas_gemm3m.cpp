@@ -3,6 +3,7 @@
 // device-pointer variants.
 #include "runtime.hpp"
 #include "gemm3m.hpp"
+#include "tuner.hpp"
 #include "../../include/gemm_amd.h"
 #include <cstdio>
 
@@ -12,18 +13,21 @@ namespace {
 
 template <typename T>
 int run3m(char ta, char tb, int m, int n, int k, const T *alpha, const T *a, int lda, const T *b, int ldb,
-          const T *beta, T *c, int ldc, hipStream_t s) {
+          const T *beta, T *c, int ldc, hipStream_t s, const GemmConfig *force = nullptr) {
     int st = 0;
     void *ws = device_workspace(WS_COMPLEX, gemm3m_workspace_bytes(m, n, k, sizeof(T)), &st);
     if (st) return st;
-    const size_t gbytes = (size_t)8 * m * n * sizeof(T);
-    void *gws = device_workspace(WS_GEMM, gbytes, &st);
+    const int nrowa = notrans(ta) ? m : k;
+    const GemmConfig cfg = force ? *force
+                                 : pick_gemm_config(sizeof(T) == 8 ? DT_C64 : DT_C32, !notrans(ta), !notrans(tb), m,
+                                                    n, k, lda - nrowa, rt().cus);
+    void *gws = device_workspace(WS_GEMM, gemm_workspace_bytes(m, n, k, cfg, (int)sizeof(T)), &st);
     if (st) return st;
     int e;
     if constexpr (sizeof(T) == 8)
-        e = launch_zgemm3m(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ws, gws, gbytes, rt().cus, s);
+        e = launch_zgemm3m(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ws, cfg, gws, s);
     else
-        e = launch_cgemm3m(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ws, gws, gbytes, rt().cus, s);
+        e = launch_cgemm3m(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, ws, cfg, gws, s);
     return hip_status((hipError_t)e, sizeof(T) == 8 ? "ZGEMM3M" : "CGEMM3M");
 }
 

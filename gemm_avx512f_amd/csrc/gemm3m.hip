@@ -83,8 +83,8 @@ size_t gemm3m_workspace_bytes(int m, int n, int k, int elem_bytes) {
 
 template <typename T>
 static int gemm3m_t(char ta, char tb, int m, int n, int k, const T *alpha, const T *A, int64_t lda, const T *B,
-                    int64_t ldb, const T *beta, T *C, int64_t ldc, void *ws, void *gemm_ws, size_t gemm_ws_bytes,
-                    int cus, hipStream_t s) {
+                    int64_t ldb, const T *beta, T *C, int64_t ldc, void *ws, const GemmConfig &cfg, void *gemm_ws,
+                    hipStream_t s) {
     if (m <= 0 || n <= 0) return 0;
     const T ar = alpha[0], ai = alpha[1], br = beta[0], bi = beta[1];
     if (k <= 0 || (ar == T(0) && ai == T(0))) {
@@ -103,10 +103,7 @@ static int gemm3m_t(char ta, char tb, int m, int n, int k, const T *alpha, const
                        pb);
     if (hipError_t e = hipGetLastError()) return (int)e;
     const size_t mk = (size_t)m * k, kn = (size_t)k * n, mn = (size_t)m * n;
-    const int dt = sizeof(T) == 8 ? 1 : 0;
     for (int p = 0; p < 3; ++p) {
-        GemmConfig cfg = pick_complex_gemm_config(dt, m, n, k, cus);
-        if (gemm_workspace_bytes(m, n, k, cfg, (int)sizeof(T)) > gemm_ws_bytes) cfg.splitk = 1;
         int e;
         if constexpr (sizeof(T) == 8)
             e = launch_dgemm(false, false, m, n, k, 1.0, pa + p * mk, m, pb + p * kn, k, 0.0, pt + p * mn, m, cfg,
@@ -122,15 +119,15 @@ static int gemm3m_t(char ta, char tb, int m, int n, int k, const T *alpha, const
 }
 
 int launch_zgemm3m(char ta, char tb, int m, int n, int k, const double *alpha, const double *A, int64_t lda,
-                   const double *B, int64_t ldb, const double *beta, double *C, int64_t ldc, void *ws, void *gemm_ws,
-                   size_t gemm_ws_bytes, int cus, hipStream_t s) {
-    return gemm3m_t<double>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ws, gemm_ws, gemm_ws_bytes, cus, s);
+                   const double *B, int64_t ldb, const double *beta, double *C, int64_t ldc, void *ws, const GemmConfig &cfg,
+                   void *gemm_ws, hipStream_t s) {
+    return gemm3m_t<double>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ws, cfg, gemm_ws, s);
 }
 
 int launch_cgemm3m(char ta, char tb, int m, int n, int k, const float *alpha, const float *A, int64_t lda,
-                   const float *B, int64_t ldb, const float *beta, float *C, int64_t ldc, void *ws, void *gemm_ws,
-                   size_t gemm_ws_bytes, int cus, hipStream_t s) {
-    return gemm3m_t<float>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ws, gemm_ws, gemm_ws_bytes, cus, s);
+                   const float *B, int64_t ldb, const float *beta, float *C, int64_t ldc, void *ws, const GemmConfig &cfg,
+                   void *gemm_ws, hipStream_t s) {
+    return gemm3m_t<float>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ws, cfg, gemm_ws, s);
 }
 
 // Pieces of the 3M method for callers that batch the three real products
@@ -155,26 +152,6 @@ int launch_gemm3m_combine_d(int m, int n, const double *t, const double *alpha, 
     hipLaunchKernelGGL(combine3m_kernel<double>, dim3(grid_of((int64_t)m * n)), dim3(256), 0, s, m, n, t, alpha[0],
                        alpha[1], beta[0], beta[1], C, ldc);
     return (int)hipGetLastError();
-}
-
-GemmConfig pick_complex_gemm_config(int dt, int m, int n, int k, int cus) {
-    GemmConfig c;
-    const long t128 = (long)((m + 127) / 128) * ((n + 127) / 128);
-    const long t64 = (long)((m + 63) / 64) * ((n + 63) / 64);
-    (void)dt;
-    if (t128 >= cus) {
-        c.tile = TILE_128x128;
-    } else {
-        c.tile = TILE_64x64;
-        if (t64 < cus && k >= 512) {
-            int want = (int)((2L * cus + t64 - 1) / t64);
-            int maxk = k / 256;
-            if (want > maxk) want = maxk;
-            if (want > 8) want = 8;
-            c.splitk = want > 1 ? want : 1;
-        }
-    }
-    return c;
 }
 
 }  // namespace gemm_amd

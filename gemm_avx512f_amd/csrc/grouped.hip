@@ -222,7 +222,7 @@ int grouped_plan_capture(void *plan, hipStream_t s) {
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(s, &g);
     if (le) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return le;
     }
     if (e != hipSuccess) return (int)e;
@@ -242,8 +242,8 @@ int grouped_plan_destroy(void *plan) {
     if (!P) return 0;
     hipError_t e = hipSuccess;
     if (P->exec) e = hipGraphExecDestroy(P->exec);
-    if (P->graph) hipGraphDestroy(P->graph);
-    if (P->d_descs) hipFree(P->d_descs);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    if (P->d_descs) (void)hipFree(P->d_descs);
     delete P;
     return (int)e;
 }

@@ -1,0 +1,281 @@
+"""Autotuner and learned configuration picker (libgemm_amd.so).
+
+* :func:`sweep` times all 20 candidate configurations (tile shape x split-K)
+  of one problem on the GPU; the sweep is exhaustive, so it is both the
+  fallback for shapes it has seen (:func:`table_set`) and the judge of the
+  picker's regret.
+* The picker is a 9-64-64-20 ReLU perceptron predicting, per candidate,
+  log2(time / best time).  It is trained on the GPU by the library's own HIP
+  kernels (:func:`train`), and run by its HIP inference kernel
+  (:func:`picker_choose`, and inside every GEMM call for unseen shapes).
+
+Command line (runs on a GPU host)::
+
+    python -m gemm_avx512f_amd.tuning sweep --seed 1 --start 0 --count 1000 --out sweep.npz
+    python -m gemm_avx512f_amd.tuning train --data a.npz b.npz --out picker.npz
+"""
+import argparse
+import ctypes
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import _lib
+
+OP_SGEMM, OP_DGEMM, OP_CGEMM3M = 0, 1, 2
+OP_NAMES = {OP_SGEMM: "sgemm", OP_DGEMM: "dgemm", OP_CGEMM3M: "cgemm3m"}
+NCFG = 20
+MLP_IN, MLP_HID, MLP_OUT = 9, 64, 20
+MLP_NPARAMS = MLP_IN * MLP_HID + MLP_HID + MLP_HID * MLP_HID + MLP_HID + MLP_HID * MLP_OUT + MLP_OUT
+TARGET_CLIP = 3.0          # targets are log2(t / t_best) clipped at 8x slower
+HERE = os.path.dirname(os.path.abspath(__file__))
+DATA = os.path.join(HERE, "data")
+WEIGHTS_INC = os.path.join(HERE, "csrc", "picker_weights.inc")
+
+SHAPE_DTYPE = np.dtype([("op", np.int32), ("ta", np.int32), ("tb", np.int32), ("m", np.int32), ("n", np.int32),
+                        ("k", np.int32), ("pad", np.int32)])
+
+
+def candidates():
+    t = (ctypes.c_int * NCFG)()
+    s = (ctypes.c_int * NCFG)()
+    n = _lib.lib().gemm_amd_tune_candidates(t, s)
+    return [(t[i], s[i]) for i in range(n)]
+
+
+def candidate_index(tile, splitk):
+    return tile * 5 + int(np.log2(splitk))
+
+
+def valid_mask(op, m, n, k):
+    """Candidates the library considers valid (each K partition >= 64 deep, partials <= 1 GiB)."""
+    m, n, k = (np.asarray(v, dtype=np.float64) for v in (m, n, k))
+    eb = np.where(np.asarray(op) == OP_DGEMM, 8.0, 4.0)
+    out = np.zeros(m.shape + (NCFG,), dtype=bool)
+    for i in range(NCFG):
+        s = 1 << (i % 5)
+        out[..., i] = True if s == 1 else (k >= 64 * s) & (m * n * s * eb <= 2.0 ** 30)
+    return out
+
+
+# ------------------------------------------------------------------ shapes
+def sample_shapes(seed, count):
+    """Seeded GEMM problems: op balanced over SGEMM/DGEMM/CGEMM3M; M, N, K
+    log-uniform over [16, 16384]; 20% with skinny K in [16, 128]; 20% with
+    M or N in [16, 64]; random transposes; lda padding 0 (half) or 1..63."""
+    rng = np.random.default_rng(seed)
+
+    def lu(lo, hi, size):
+        return np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size))).astype(np.int32)
+
+    s = np.zeros(count, dtype=SHAPE_DTYPE)
+    s["op"] = np.arange(count) % 3
+    s["m"], s["n"], s["k"] = lu(16, 16384, count), lu(16, 16384, count), lu(16, 16384, count)
+    kind = rng.uniform(size=count)
+    skinny = kind < 0.2
+    s["k"][skinny] = lu(16, 128, int(skinny.sum()))
+    small = (kind >= 0.2) & (kind < 0.4)
+    which = rng.integers(0, 2, count)
+    sm_m, sm_n = small & (which == 0), small & (which == 1)
+    s["m"][sm_m] = lu(16, 64, int(sm_m.sum()))
+    s["n"][sm_n] = lu(16, 64, int(sm_n.sum()))
+    s["ta"], s["tb"] = rng.integers(0, 2, count), rng.integers(0, 2, count)
+    pad = rng.integers(1, 64, count)
+    s["pad"] = np.where(rng.uniform(size=count) < 0.5, 0, pad)
+    return s
+
+
+def features(shapes):
+    """The network's inputs (same formula as the library's mlp_features)."""
+    s = shapes
+    f = np.zeros((len(s), MLP_IN), dtype=np.float64)
+    f[:, 0] = (np.log2(np.maximum(s["m"], 1)) - 9.0) / 5.0
+    f[:, 1] = (np.log2(np.maximum(s["n"], 1)) - 9.0) / 5.0
+    f[:, 2] = (np.log2(np.maximum(s["k"], 1)) - 9.0) / 5.0
+    f[:, 3] = s["ta"]
+    f[:, 4] = s["tb"]
+    for o in range(3):
+        f[:, 5 + o] = s["op"] == o
+    f[:, 8] = (s["pad"] % 64) / 64.0
+    return f.astype(np.float32)
+
+
+def targets(ms):
+    """log2(t_c / t_best) clipped at TARGET_CLIP, and the mask of measured candidates."""
+    ms = np.asarray(ms, dtype=np.float64)[:, :NCFG]
+    ok = ms > 0
+    best = np.min(np.where(ok, ms, np.inf), axis=1, keepdims=True)
+    t = np.where(ok, np.minimum(np.log2(np.where(ok, ms, 1.0) / best), TARGET_CLIP), 0.0)
+    return t.astype(np.float32), ok.astype(np.float32)
+
+
+def regret(ms, choice):
+    """t(choice) / t(best) - 1 per shape (choice = candidate index, or NCFG for the heuristic slot)."""
+    ms = np.asarray(ms, dtype=np.float64)
+    best = np.min(np.where(ms[:, :NCFG] > 0, ms[:, :NCFG], np.inf), axis=1)
+    picked = ms[np.arange(len(ms)), choice]
+    return picked / best - 1.0
+
+
+# ------------------------------------------------------------------ GPU sweep
+def sweep(op, ta, tb, m, n, k, pad=0):
+    ms = (ctypes.c_float * (NCFG + 1))()
+    rc = _lib.lib().gemm_amd_tune_sweep(op, b"T" if ta else b"N", b"T" if tb else b"N", m, n, k, pad, ms)
+    _lib.check(rc, "tune_sweep")
+    return np.array(ms[:], dtype=np.float32)
+
+
+def sweep_shapes(shapes, log=None, every=100):
+    out = np.zeros((len(shapes), NCFG + 1), dtype=np.float32)
+    t0 = time.time()
+    for i, s in enumerate(shapes):
+        out[i] = sweep(int(s["op"]), bool(s["ta"]), bool(s["tb"]), int(s["m"]), int(s["n"]), int(s["k"]),
+                       int(s["pad"]))
+        if log is not None and (i + 1) % every == 0:
+            log("swept %d/%d shapes in %.1f s" % (i + 1, len(shapes), time.time() - t0))
+    return out
+
+
+# ------------------------------------------------------------------ picker
+def _shape_args(shapes):
+    s = shapes
+    cnt = len(s)
+    ints = lambda a: (ctypes.c_int * max(cnt, 1))(*[int(v) for v in a])  # noqa: E731
+    ta = ctypes.create_string_buffer(b"".join(b"T" if v else b"N" for v in s["ta"]) or b"N")
+    tb = ctypes.create_string_buffer(b"".join(b"T" if v else b"N" for v in s["tb"]) or b"N")
+    return [ints(s["op"]), ta, tb, ints(s["m"]), ints(s["n"]), ints(s["k"]), ints(s["pad"])]
+
+
+def picker_load(params):
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    _lib.check(_lib.lib().gemm_amd_picker_load(p.ctypes.data_as(ctypes.c_void_p), p.size), "picker_load")
+
+
+def picker_set_mode(mode):
+    _lib.check(_lib.lib().gemm_amd_picker_set_mode(mode), "picker_set_mode")
+
+
+def picker_choose(shapes):
+    """Candidate index chosen by the picker (HIP inference) for each shape."""
+    args = _shape_args(shapes)
+    out = (ctypes.c_int * max(len(shapes), 1))()
+    _lib.check(_lib.lib().gemm_amd_picker_choose(len(shapes), *args, out), "picker_choose")
+    return np.array(out[: len(shapes)], dtype=np.int64)
+
+
+def pick_config(op, ta, tb, m, n, k, pad=0, mode=2):
+    t, s = ctypes.c_int(), ctypes.c_int()
+    _lib.check(_lib.lib().gemm_amd_pick_config(op, b"T" if ta else b"N", b"T" if tb else b"N", m, n, k, pad, mode,
+                                               ctypes.byref(t), ctypes.byref(s)), "pick_config")
+    return t.value, s.value
+
+
+def table_set(shapes, ms):
+    """Install the sweep's winners for these exact shapes (they override the picker)."""
+    ms = np.asarray(ms)
+    choice = np.argmin(np.where(ms[:, :NCFG] > 0, ms[:, :NCFG], np.inf), axis=1)
+    args = _shape_args(shapes)
+    ch = (ctypes.c_int * max(len(shapes), 1))(*[int(c) for c in choice])
+    _lib.check(_lib.lib().gemm_amd_tune_table_set(len(shapes), *args, ch), "tune_table_set")
+
+
+def table_clear():
+    _lib.lib().gemm_amd_tune_table_clear()
+
+
+# ------------------------------------------------------------------ training (GPU kernels)
+def init_params(seed):
+    """He-uniform weights, zero biases (float32, the library's flat layout)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fan_in, fan_out in ((MLP_IN, MLP_HID), (MLP_HID, MLP_HID), (MLP_HID, MLP_OUT)):
+        lim = np.sqrt(6.0 / fan_in)
+        parts += [rng.uniform(-lim, lim, size=fan_in * fan_out), np.zeros(fan_out)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def train(x, t, mask, epochs=300, batch=256, lr=2e-3, seed=0, params=None):
+    """Train on the GPU with the library's HIP kernels; returns (params, per-epoch loss)."""
+    from .device import DeviceBuffer
+    x, t, mask = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, t, mask))
+    p0 = init_params(seed) if params is None else np.ascontiguousarray(params, dtype=np.float32)
+    bufs = {}
+    for name, arr in (("p", p0), ("m", np.zeros_like(p0)), ("v", np.zeros_like(p0)), ("x", x), ("t", t),
+                      ("mask", mask)):
+        bufs[name] = DeviceBuffer(arr.nbytes)
+        bufs[name].upload(arr)
+    losses = np.zeros(epochs, dtype=np.float32)
+    rc = _lib.lib().gemm_amd_mlp_train(bufs["p"].ptr, bufs["m"].ptr, bufs["v"].ptr, bufs["x"].ptr, bufs["t"].ptr,
+                                       bufs["mask"].ptr, len(x), batch, epochs, lr, seed,
+                                       losses.ctypes.data_as(ctypes.c_void_p))
+    _lib.check(rc, "mlp_train")
+    out = bufs["p"].download(np.empty_like(p0))
+    return out, losses
+
+
+def write_weights_inc(params, path=WEIGHTS_INC, note=""):
+    """Write the trained parameters as the C include compiled into the library."""
+    p = np.asarray(params, dtype=np.float32)
+    assert p.size == MLP_NPARAMS
+    lines = ["// Trained weights of the configuration picker (flat layout of mlp.hpp),",
+             "// written by gemm_avx512f_amd/tuning.py." + (" " + note if note else ""),
+             "static const float kPickerWeights[%d] = {" % p.size]
+    vals = ["%.8ef" % float(v) if np.isfinite(v) else "0.0f" for v in p]   # 9 digits round-trip FP32
+    for i in range(0, len(vals), 8):
+        lines.append("    " + ", ".join(vals[i:i + 8]) + ",")
+    lines += ["};", "static const int kPickerWeightsCount = %d;" % p.size, ""]
+    with open(path, "w") as f:
+        f.write("\n".join(lines))
+
+
+def load_dataset(paths):
+    shapes, ms = [], []
+    for p in paths:
+        d = np.load(p, allow_pickle=False)
+        shapes.append(d["shapes"])
+        ms.append(d["ms"])
+    return np.concatenate(shapes), np.concatenate(ms)
+
+
+# ------------------------------------------------------------------ command line
+def _main(argv):
+    ap = argparse.ArgumentParser(prog="python -m gemm_avx512f_amd.tuning")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("sweep")
+    s.add_argument("--seed", type=int, required=True)
+    s.add_argument("--total", type=int, default=None, help="size of the seeded shape list (default start+count)")
+    s.add_argument("--start", type=int, default=0)
+    s.add_argument("--count", type=int, required=True)
+    s.add_argument("--out", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--data", nargs="+", required=True)
+    t.add_argument("--out", required=True)
+    t.add_argument("--epochs", type=int, default=300)
+    t.add_argument("--batch", type=int, default=256)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--seed", type=int, default=0)
+    a = ap.parse_args(argv)
+    log = lambda msg: print(msg, flush=True)  # noqa: E731
+    if a.cmd == "sweep":
+        total = a.total or (a.start + a.count)
+        shapes = sample_shapes(a.seed, total)[a.start:a.start + a.count]
+        log("sweeping %d shapes (seed %d, %d..%d)" % (len(shapes), a.seed, a.start, a.start + len(shapes)))
+        ms = sweep_shapes(shapes, log=log)
+        np.savez_compressed(a.out, shapes=shapes, ms=ms, seed=a.seed, start=a.start)
+        log("wrote " + a.out)
+    else:
+        shapes, ms = load_dataset(a.data)
+        x = features(shapes)
+        tt, mask = targets(ms)
+        t0 = time.time()
+        params, losses = train(x, tt, mask, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed)
+        log("trained %d epochs on %d shapes in %.1f s; loss %.4f -> %.4f" % (a.epochs, len(x), time.time() - t0,
+                                                                           losses[0], losses[-1]))
+        np.savez_compressed(a.out, params=params, losses=losses)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(_main(sys.argv[1:]))

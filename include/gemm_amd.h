@@ -173,6 +173,44 @@ int gemm_amd_zgemm3m_batched(int count, const char *transa, const char *transb, 
                              const double *const *b, const int *ldb, const double *beta, double *const *c,
                              const int *ldc, void *stream);
 
+/* ============================================ Part 2b: autotuner and picker
+ * Replaces the reference's compile-time tuning macros (sgemm_tune.h:6-15,
+ * dgemm_tune.h:6-15, src/gemm_set_parameters.h:66-82) with run-time
+ * configuration choice.  op: 0 = SGEMM, 1 = DGEMM, 2 = CGEMM3M.  The 20
+ * candidates are tile shape (0..3) x split-K (1, 2, 4, 8, 16), index
+ * tile*5 + log2(splitk).  lda_pad = lda - rows of A's storage. */
+int gemm_amd_tune_candidates(int *tile, int *splitk);   /* returns 20 */
+/* Times every valid candidate on library-owned seeded inputs (alpha 1, beta 0):
+ * ms[i] for candidate i (-1 when invalid), ms[20] for the built-in heuristic's
+ * own choice.  Best of 2-3 batches after a warm-up run. */
+int gemm_amd_tune_sweep(int op, char transa, char transb, int m, int n, int k, int lda_pad, float *ms);
+/* mode 0 = heuristic, 1 = learned picker, 2 = the library's own order
+ * (environment override, sweep table, picker when enabled, heuristic). */
+int gemm_amd_pick_config(int op, char transa, char transb, int m, int n, int k, int lda_pad, int mode, int *tile,
+                         int *splitk);
+/* The learned picker: a 9-64-64-20 ReLU perceptron (6100 FP32 parameters,
+ * flat layout W1 b1 W2 b2 W3 b3, weights [in][out] row-major) run by a HIP
+ * inference kernel.  Trained weights are compiled in; _load replaces them. */
+int gemm_amd_picker_load(const float *params, int nparams);
+int gemm_amd_picker_set_mode(int mode);   /* 0 heuristic, 1 learned picker */
+int gemm_amd_picker_choose(int count, const int *op, const char *transa, const char *transb, const int *m,
+                           const int *n, const int *k, const int *lda_pad, int *choice);
+/* Exact-shape results of the exhaustive sweep; they take precedence over the picker. */
+int gemm_amd_tune_table_set(int count, const int *op, const char *transa, const char *transb, const int *m,
+                            const int *n, const int *k, const int *lda_pad, const int *choice);
+int gemm_amd_tune_table_clear(void);
+/* Training kernels (device pointers, row-major [batch][9] inputs, [batch][20]
+ * targets and masks).  loss = sum(mask*(y-t)^2)/(batch*20). */
+int gemm_amd_mlp_forward(const float *params, const float *x, int batch, float *y, void *stream);
+int gemm_amd_mlp_grad(const float *params, const float *x, const float *t, const float *mask, int batch, float *grad,
+                      float *loss, void *stream);
+int gemm_amd_mlp_adam(float *params, const float *grad, float *m, float *v, int n, float lr, float beta1, float beta2,
+                      float eps, int step, void *stream);
+/* Minibatch Adam over `epochs` shuffled passes (cosine step-size decay);
+ * epoch_loss (host, `epochs` floats) receives each epoch's mean batch loss. */
+int gemm_amd_mlp_train(float *params, float *adam_m, float *adam_v, const float *x, const float *t, const float *mask,
+                       int nsamples, int batch, int epochs, float lr, uint64_t seed, float *epoch_loss);
+
 /* ===================================================== Part 3: runtime */
 
 int gemm_amd_init(void);                         /* 0 or negative HIP error; idempotent */

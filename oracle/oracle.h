@@ -79,6 +79,19 @@ void oracle_bf16_split(int64_t n, const float *x, int nsplit, uint16_t *parts /*
 void oracle_dgemm_blocked(char transa, char transb, int m, int n, int k, double alpha,
                           const double *a, int lda, const double *b, int ldb,
                           double beta, double *c, int ldc, int nthreads);
+
+/* ---- configuration picker's perceptron, in double (mlp_oracle.c) ---- */
+#define ORACLE_MLP_IN 9
+#define ORACLE_MLP_HID 64
+#define ORACLE_MLP_OUT 20
+#define ORACLE_MLP_NPARAMS (ORACLE_MLP_IN * ORACLE_MLP_HID + ORACLE_MLP_HID + ORACLE_MLP_HID * ORACLE_MLP_HID + \
+                            ORACLE_MLP_HID + ORACLE_MLP_HID * ORACLE_MLP_OUT + ORACLE_MLP_OUT)
+void oracle_mlp_forward(const double *p, const double *x, int batch, double *y);
+double oracle_mlp_loss(const double *p, const double *x, const double *t, const double *mask, int batch);
+void oracle_mlp_grad(const double *p, const double *x, const double *t, const double *mask, int batch, double *g,
+                     double *loss);
+void oracle_adam(double *p, const double *g, double *m, double *v, int n, double lr, double b1, double b2,
+                 double eps, int step);
 #ifdef __cplusplus
 }
 #endif

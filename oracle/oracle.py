@@ -207,6 +207,55 @@ def dgemm_blocked(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nthreads
     lib().oracle_dgemm_blocked(_ch(ta), _ch(tb), m, n, k, alpha, _p(a), lda, _p(b), ldb, beta, _p(c), ldc, nthreads)
 
 
+# ------------------------------------------------------------------ picker perceptron (mlp_oracle.c)
+MLP_IN, MLP_HID, MLP_OUT = 9, 64, 20
+MLP_NPARAMS = MLP_IN * MLP_HID + MLP_HID + MLP_HID * MLP_HID + MLP_HID + MLP_HID * MLP_OUT + MLP_OUT
+
+
+def _mlp_sig():
+    L = lib()
+    if not hasattr(L, "_mlp_ready"):
+        L.oracle_mlp_forward.argtypes = [_vp, _vp, _i, _vp]
+        L.oracle_mlp_forward.restype = None
+        L.oracle_mlp_loss.argtypes = [_vp, _vp, _vp, _vp, _i]
+        L.oracle_mlp_loss.restype = _d
+        L.oracle_mlp_grad.argtypes = [_vp, _vp, _vp, _vp, _i, _vp, _vp]
+        L.oracle_mlp_grad.restype = None
+        L.oracle_adam.argtypes = [_vp, _vp, _vp, _vp, _i, _d, _d, _d, _d, _i]
+        L.oracle_adam.restype = None
+        L._mlp_ready = True
+    return L
+
+
+def _d64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def mlp_forward(params, x):
+    p, x = _d64(params), _d64(x)
+    y = np.zeros((x.shape[0], MLP_OUT))
+    _mlp_sig().oracle_mlp_forward(_p(p), _p(x), x.shape[0], _p(y))
+    return y
+
+
+def mlp_loss(params, x, t, mask):
+    p, x, t, mask = _d64(params), _d64(x), _d64(t), _d64(mask)
+    return _mlp_sig().oracle_mlp_loss(_p(p), _p(x), _p(t), _p(mask), x.shape[0])
+
+
+def mlp_grad(params, x, t, mask):
+    p, x, t, mask = _d64(params), _d64(x), _d64(t), _d64(mask)
+    g = np.zeros(MLP_NPARAMS)
+    loss = ctypes.c_double()
+    _mlp_sig().oracle_mlp_grad(_p(p), _p(x), _p(t), _p(mask), x.shape[0], _p(g), ctypes.byref(loss))
+    return g, loss.value
+
+
+def adam(params, g, m, v, lr, b1, b2, eps, step):
+    """One Adam step in place on float64 arrays."""
+    _mlp_sig().oracle_adam(_p(params), _p(_d64(g)), _p(m), _p(v), params.size, lr, b1, b2, eps, step)
+
+
 # ------------------------------------------------------------------ reference build (oracle/_ref)
 _ref_cache = {}
 
