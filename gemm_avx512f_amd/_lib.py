@@ -54,6 +54,11 @@ SIGNATURES = {
                                      c_double, c_vp, c_int, c_int, c_vp]),
     "gemm_amd_ozaki_split": (c_int, [c_vp, ctypes.c_int64, ctypes.c_int64, c_int, c_int, c_int, c_vp, c_int, c_int,
                                      c_vp, c_vp]),
+    "gemm_amd_dgemm_ozaki_crt": (c_int, [c_char, c_char, c_int, c_int, c_int, c_double, c_vp, c_int, c_vp, c_int,
+                                         c_double, c_vp, c_int, c_int, c_vp]),
+    "gemm_amd_ozaki_crt_bits": (c_int, [c_int, c_int]),
+    "gemm_amd_ozaki_crt_split": (c_int, [c_vp, ctypes.c_int64, ctypes.c_int64, c_int, c_int, c_int, c_int, c_vp,
+                                         c_int, c_int, c_vp, c_vp]),
     "gemm_amd_sgemm_bf16x": (c_int, [c_char, c_char, c_int, c_int, c_int, c_float, c_vp, c_int, c_vp, c_int,
                                      c_float, c_vp, c_int, c_int, c_vp]),
     "gemm_amd_bf16_split": (c_int, [c_vp, ctypes.c_int64, ctypes.c_int64, c_int, c_int, c_int, c_vp, c_int, c_int,

@@ -157,6 +157,14 @@ def dgemm_ozaki(transa, transb, m, n, k, alpha, a: DeviceMatrix, b: DeviceMatrix
     return _lib.check(rc, "dgemm_ozaki")
 
 
+def dgemm_ozaki_crt(transa, transb, m, n, k, alpha, a: DeviceMatrix, b: DeviceMatrix, beta, c: DeviceMatrix,
+                    nmoduli=16, stream=None):
+    """DGEMM emulated on int8 matrix cores in modular form: one int8 GEMM per modulus (2..16), CRT rebuild."""
+    rc = _lib.lib().gemm_amd_dgemm_ozaki_crt(_enc(transa), _enc(transb), m, n, k, alpha, a.ptr, a.ld, b.ptr, b.ld,
+                                             beta, c.ptr, c.ld, nmoduli, _stream(stream))
+    return _lib.check(rc, "dgemm_ozaki_crt")
+
+
 def sgemm_bf16x(transa, transb, m, n, k, alpha, a: DeviceMatrix, b: DeviceMatrix, beta, c: DeviceMatrix, nprod=3,
                 stream=None):
     """SGEMM on bf16 matrix cores from exact bf16 splits: nprod=3 (bf16x3) or 6 (bf16x6)."""

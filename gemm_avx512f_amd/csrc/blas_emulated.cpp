@@ -94,3 +94,61 @@ extern "C" int gemm_amd_ozaki_split(const double *x, int64_t row_stride, int64_t
                                                           k_pad, exps, (unsigned long long *)rm, (hipStream_t)stream),
                       "ozaki_split");
 }
+
+extern "C" int gemm_amd_dgemm_ozaki_crt(char ta, char tb, int m, int n, int k, double alpha, const double *a, int lda,
+                                        const double *b, int ldb, double beta, double *c, int ldc, int nmoduli,
+                                        void *stream) {
+    const int info = gemm_check(ta, tb, m, n, k, lda, ldb, ldc);
+    if (info) {
+        xerbla("DGEMM_OZAKI_CRT", info);
+        return info;
+    }
+    if (nmoduli < 2 || nmoduli > OZ_CRT_MAXMOD) {
+        set_error("gemm_amd_dgemm_ozaki_crt: nmoduli must be 2..16");
+        return -(int)hipErrorInvalidValue;
+    }
+    if (m == 0 || n == 0 || ((alpha == 0.0 || k == 0) && beta == 1.0)) return 0;
+    if (ozaki_crt_bits(nmoduli, k) < 1) {
+        set_error("gemm_amd_dgemm_ozaki_crt: too few moduli for this K (no bits left per operand)");
+        return -(int)hipErrorInvalidValue;
+    }
+    if (int st = init_status()) return st;
+    hipStream_t s = (hipStream_t)stream;
+    if (alpha == 0.0 || k == 0)
+        return hip_status((hipError_t)launch_scale_d(m, n, beta, c, ldc, s), "DGEMM_OZAKI_CRT");
+    int st = 0;
+    void *wa = device_workspace(WS_EMU_A, ozaki_crt_workspace_bytes(m, k, nmoduli), &st);
+    if (st) return st;
+    void *wb = device_workspace(WS_EMU_B, ozaki_crt_workspace_bytes(n, k, nmoduli), &st);
+    if (st) return st;
+    void *ww = device_workspace(WS_EMU_W, ozaki_crt_accum_bytes(m, n, nmoduli), &st);
+    if (st) return st;
+    const int e = launch_dgemm_ozaki_crt(!notrans(ta), !notrans(tb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                                         nmoduli, wa, wb, ww, s);
+    return hip_status((hipError_t)e, "DGEMM_OZAKI_CRT");
+}
+
+extern "C" int gemm_amd_ozaki_crt_bits(int nmoduli, int k) {
+    if (nmoduli < 2 || nmoduli > OZ_CRT_MAXMOD || k < 1) return -1;
+    return ozaki_crt_bits(nmoduli, k);
+}
+
+extern "C" int gemm_amd_ozaki_crt_split(const double *x, int64_t row_stride, int64_t k_stride, int rows, int k,
+                                        int nmoduli, int bits, int8_t *residues, int rows_pad, int k_pad, int *exps,
+                                        void *stream) {
+    if (rows < 0 || k < 0 || nmoduli < 2 || nmoduli > OZ_CRT_MAXMOD || bits < 1 || bits > 56 || rows_pad < rows ||
+        k_pad < k || rows_pad % 64 != 0 || k_pad % 64 != 0) {
+        set_error("gemm_amd_ozaki_crt_split: need rows, k >= 0, nmoduli in 2..16, bits in 1..56, rows_pad >= rows "
+                  "and k_pad >= k, both multiples of 64");
+        return -(int)hipErrorInvalidValue;
+    }
+    if (rows == 0 || k == 0) return 0;
+    if (int st = init_status()) return st;
+    int st = 0;
+    void *rm = device_workspace(WS_EMU_AUX, (size_t)rows_pad * 8, &st);
+    if (st) return st;
+    return hip_status((hipError_t)launch_ozaki_crt_split_only(x, row_stride, k_stride, rows, k, nmoduli, bits,
+                                                              residues, rows_pad, k_pad, exps,
+                                                              (unsigned long long *)rm, (hipStream_t)stream),
+                      "ozaki_crt_split");
+}

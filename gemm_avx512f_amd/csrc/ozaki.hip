@@ -26,7 +26,9 @@
 // the split, the flush and the epilogue.
 #include "common.hpp"
 #include "ozaki.hpp"
+#include <cmath>
 #include <cstdlib>
+#include <mutex>
 
 namespace gemm_amd {
 
@@ -402,6 +404,453 @@ int launch_ozaki_split_only(const double *X, int64_t rs, int64_t ks, int R, int 
                        rowmax_ws);
     hipLaunchKernelGGL(oz_slice_kernel, dim3(Kp / 64, Rp / 64), dim3(256), 0, s, X, rs, ks, R, K, Rp, Kp, ns,
                        rowmax_ws, slices, exps);
+    return (int)hipGetLastError();
+}
+
+}  // namespace gemm_amd
+
+// ============================================================================
+// Emulated DGEMM on the int8 matrix cores, modular (CRT) form.
+//
+// Method: Ozaki, Uchino and Imamura, "Ozaki Scheme II: A GEMM-oriented
+// emulation of floating-point matrix multiplication using an integer modular
+// technique" (2025).  No reference counterpart.
+//
+//   scale:   every row i of op(A) becomes integers A'_ik = rint(a_ik 2^(al-e_i))
+//            (e_i from frexp of the row's max |a|, so |A'| <= 2^al); columns
+//            of op(B) likewise with f_j.  al is the largest value with
+//            K 2^(2 al) <= M/4, M = m_1 ... m_N (pairwise coprime moduli
+//            <= 256), so the exact integer product V = A'B' has |V| <= M/4.
+//   residues: A'_ik mod m_l and B'_kj mod m_l in symmetric form fit int8.
+//   products: one int8 GEMM per modulus with exact int32 accumulation
+//            (|r| <= 128: exact for 131008 k terms, longer K is reduced in
+//            between), reduced mod m_l to an int8 residue of V.
+//   CRT:     V = sum_l r_l W_l - q M with W_l = (M/m_l) ((M/m_l)^-1 mod m_l);
+//            W_l and M are split into four 32-bit limbs, so the limb sums
+//            s_k = sum_l r_l w_lk (< 2^43) and s_k - q M_k are exact in FP64,
+//            q = rint(s/M) is exact because |V|/M <= 1/4, and a carry pass
+//            in int64 rebuilds V, rounded once or twice to FP64.
+//   C_ij = alpha 2^(e_i + f_j - 2 al) V_ij + beta C_ij.
+// N moduli cost N int8 GEMMs (15 or 16 give FP64-level accuracy at K = 8192)
+// against s(s+1)/2 - 1 for s slices.
+// ============================================================================
+
+namespace gemm_amd {
+
+struct CrtConst {
+    int nmod, pad;
+    int m[OZ_CRT_MAXMOD];
+    double invm[OZ_CRT_MAXMOD];
+    double w[OZ_CRT_MAXMOD][4];   // limbs of W_l, low first, each < 2^32
+    double Mk[4];                 // limbs of M
+    double invM;                  // 1 / M (rounded)
+};
+
+static constexpr int OZ_CRT_CHUNK_K = 131008;   // 131008 * 128^2 < 2^31
+
+// grid (Kp/64, Rp/64): residues [l][Rp][Kp] (zero padding) and exps[r] = e_r - al.
+__global__ __launch_bounds__(256) void oz_residue_kernel(const double *__restrict__ X, int64_t rs, int64_t ks, int R,
+                                                         int K, int Rp, int Kp, int al,
+                                                         const unsigned long long *__restrict__ rowmax,
+                                                         const CrtConst *__restrict__ cc, int8_t *__restrict__ out,
+                                                         int *__restrict__ exps) {
+    __shared__ double T[64][65];
+    const int r0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
+    oz_load_tile(X, rs, ks, R, K, r0, k0, T);
+    __syncthreads();
+    const int tid = threadIdx.x, rr = tid >> 2, kq = (tid & 3) * 16;
+    const double amax = __longlong_as_double((long long)rowmax[r0 + rr]);
+    int e = 0;
+    if (amax > 0.0) frexp(amax, &e);
+    if (blockIdx.x == 0 && (tid & 3) == 0) exps[r0 + rr] = e - al;
+    double x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) x[i] = rint(ldexp(T[rr][kq + i], al - e));   // |x| <= 2^al, exact integer
+    const int64_t plane = (int64_t)Rp * Kp;
+    int8_t *dst = out + (int64_t)(r0 + rr) * Kp + k0 + kq;
+    const int nmod = cc->nmod;
+    for (int l = 0; l < nmod; ++l) {
+        const double m = (double)cc->m[l], im = cc->invm[l], h = 0.5 * m;
+        i4_t w = {0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            // x - m*rint(x/m) is exact (fma); rint(x/m) can be off by one for
+            // |x| near 2^56, which one correction absorbs
+            double r = fma(-rint(x[i] * im), m, x[i]);
+            r = r > h ? r - m : (r < -h ? r + m : r);
+            w[i >> 2] |= ((int)r & 0xFF) << (8 * (i & 3));
+        }
+        *reinterpret_cast<i4_t *>(dst + l * plane) = w;
+    }
+}
+
+struct CrtParams {
+    int M, N, Kp, tiles_m, tiles_n;
+    int64_t planeA, planeB;       // bytes per residue plane
+    const int8_t *RA, *RB;
+    const int *ea, *eb;
+    const CrtConst *cc;
+    int *Wi;                      // int32 staging of the accumulators (per wave, fragment order)
+    unsigned *Rv;                 // product residues, 4 per word, per wave [l][idx/4][lane]
+    double alpha, beta;
+    double *C;
+    int64_t ldc;
+};
+
+// Symmetric residue of an int32 mod m (exact: |x| < 2^31, FP64 quotient).
+__device__ __forceinline__ int crt_mod(int x, int m, double im) {
+    const int q = (int)rint((double)x * im);
+    int r = x - q * m;
+    const int h = m >> 1;
+    r = r > h ? r - m : (r < -h ? r + m : r);
+    return r;
+}
+
+template <int WM, int WN, int TM, int TN, int KB>
+__global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
+    constexpr int KP = KB + 16;
+    constexpr int NT = WM * WN * 64;
+    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
+    constexpr int STAGE = (BM + BN) * KP;
+    constexpr int CH = KB / 16;
+    constexpr int PA = BM * CH / NT, PB = BN * CH / NT;
+    constexpr int NACC = TM * TN * 16;            // int32 accumulators per lane
+    static_assert(BM * CH % NT == 0 && BN * CH % NT == 0, "stage must split evenly over the workgroup");
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm0 = (wave % WM) * (TM * 32), wn0 = (wave / WM) * (TN * 32);
+    int tm, tn;
+    tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, 8, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int nkt = p.Kp / KB;
+    const int nmod = p.cc->nmod;
+    const int total = nmod * nkt;
+    const int64_t wv = (int64_t)(tm * p.tiles_n + tn) * (WM * WN) + wave;
+    int *Wi = p.Wi + wv * (NACC * 64);
+    unsigned *Rv = p.Rv + wv * ((int64_t)nmod * (NACC / 4) * 64);
+
+    i4_t ra[PA], rb[PB];
+    auto gload = [&](int st) {
+        const int l = st / nkt, kt = st - l * nkt;
+        const int8_t *a = p.RA + (int64_t)l * p.planeA + (int64_t)m0 * p.Kp + kt * KB;
+        const int8_t *b = p.RB + (int64_t)l * p.planeB + (int64_t)n0 * p.Kp + kt * KB;
+#pragma unroll
+        for (int i = 0; i < PA; ++i) {
+            const int v = tid + i * NT, row = v / CH, ch = (v % CH) * 16;
+            ra[i] = *reinterpret_cast<const i4_t *>(a + (int64_t)row * p.Kp + ch);
+        }
+#pragma unroll
+        for (int i = 0; i < PB; ++i) {
+            const int v = tid + i * NT, row = v / CH, ch = (v % CH) * 16;
+            rb[i] = *reinterpret_cast<const i4_t *>(b + (int64_t)row * p.Kp + ch);
+        }
+    };
+    auto sstore = [&](unsigned char *buf) {
+#pragma unroll
+        for (int i = 0; i < PA; ++i) {
+            const int v = tid + i * NT, row = v / CH, ch = (v % CH) * 16;
+            *reinterpret_cast<i4_t *>(buf + row * KP + ch) = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < PB; ++i) {
+            const int v = tid + i * NT, row = v / CH, ch = (v % CH) * 16;
+            *reinterpret_cast<i4_t *>(buf + (BM + row) * KP + ch) = rb[i];
+        }
+    };
+
+    const int q = lane & 31, g = lane >> 5;
+    const int aoff = (wm0 + q) * KP + 16 * g;
+    const int boff = (BM + wn0 + q) * KP + 16 * g;
+    i16_t acc[TM][TN];
+
+    if (total > 0) {
+        gload(0);
+        sstore(smem);
+    }
+    __syncthreads();
+    constexpr int CHUNK = OZ_CRT_CHUNK_K / KB;
+    int st = 0;
+    for (int l = 0; l < nmod; ++l) {
+        const int m = p.cc->m[l];
+        const double im = p.cc->invm[l];
+        bool carry = false;   // Wi holds a reduced partial sum of this modulus
+        for (int c0 = 0; c0 < nkt; c0 += CHUNK) {
+            const int c1 = min(nkt, c0 + CHUNK);
+#pragma unroll
+            for (int a = 0; a < TM; ++a)
+#pragma unroll
+                for (int b = 0; b < TN; ++b)
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) acc[a][b][e] = 0;
+            for (int i = c0; i < c1; ++i, ++st) {
+                const unsigned char *cur = smem + (st & 1) * STAGE;
+                unsigned char *nxt = smem + ((st + 1) & 1) * STAGE;
+                if (st + 1 < total) gload(st + 1);
+#pragma unroll
+                for (int ks = 0; ks < KB / 32; ++ks) {
+                    i4_t av[TM], bv[TN];
+#pragma unroll
+                    for (int s = 0; s < TM; ++s)
+                        av[s] = *reinterpret_cast<const i4_t *>(cur + aoff + 32 * s * KP + 32 * ks);
+#pragma unroll
+                    for (int s = 0; s < TN; ++s)
+                        bv[s] = *reinterpret_cast<const i4_t *>(cur + boff + 32 * s * KP + 32 * ks);
+#pragma unroll
+                    for (int a = 0; a < TM; ++a)
+#pragma unroll
+                        for (int b = 0; b < TN; ++b)
+                            acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[a], bv[b], acc[a][b], 0, 0, 0);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if (st + 1 < total) sstore(nxt);
+                __syncthreads();
+            }
+            // the accumulators leave through plain int32 stores (an opaque
+            // base keeps the store addresses from being hoisted into VGPRs)
+            int *wb = Wi + lane;
+            __asm__ volatile("" : "+v"(wb));
+#pragma unroll
+            for (int a = 0; a < TM; ++a)
+#pragma unroll
+                for (int b = 0; b < TN; ++b)
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        int *dst = wb + ((a * TN + b) * 16 + e) * 64;
+                        *dst = carry ? acc[a][b][e] + *dst : acc[a][b][e];
+                    }
+            if (c1 < nkt) {
+                // K longer than one exact chunk: fold the partial sum to a residue
+                for (int idx = 0; idx < NACC; ++idx) Wi[idx * 64 + lane] = crt_mod(Wi[idx * 64 + lane], m, im);
+                carry = true;
+            }
+        }
+        // residues of this modulus, four per word, in accumulator order
+        for (int w4 = 0; w4 < NACC / 4; ++w4) {
+            unsigned word = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                word |= (unsigned)(crt_mod(Wi[(w4 * 4 + e) * 64 + lane], m, im) & 0xFF) << (8 * e);
+            Rv[((int64_t)l * (NACC / 4) + w4) * 64 + lane] = word;
+        }
+    }
+
+    // CRT reconstruction (a memory loop over the wave's residues)
+    const double two32 = 4294967296.0;
+    for (int w4 = 0; w4 < NACC / 4; ++w4) {
+        double s[4][4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) s[e][k] = 0.0;
+        for (int l = 0; l < nmod; ++l) {
+            const unsigned word = Rv[((int64_t)l * (NACC / 4) + w4) * 64 + lane];
+            const double w0 = p.cc->w[l][0], w1 = p.cc->w[l][1], w2 = p.cc->w[l][2], w3 = p.cc->w[l][3];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const double r = (double)(int)(signed char)(word >> (8 * e));
+                s[e][0] = fma(r, w0, s[e][0]);
+                s[e][1] = fma(r, w1, s[e][1]);
+                s[e][2] = fma(r, w2, s[e][2]);
+                s[e][3] = fma(r, w3, s[e][3]);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int idx = w4 * 4 + e;
+            const int ab = idx >> 4, ee = idx & 15, a = ab / TN, b = ab % TN;
+            const int i = m0 + wm0 + 32 * a + (ee >> 2) * 8 + g * 4 + (ee & 3);
+            const int j = n0 + wn0 + 32 * b + q;
+            if (i >= p.M || j >= p.N) continue;
+            const double approx = (s[e][3] * two32 + s[e][2]) * (two32 * two32) + s[e][1] * two32;
+            const double qq = rint(approx * p.cc->invM);
+            long long t0 = (long long)fma(-qq, p.cc->Mk[0], s[e][0]);
+            long long t1 = (long long)fma(-qq, p.cc->Mk[1], s[e][1]);
+            long long t2 = (long long)fma(-qq, p.cc->Mk[2], s[e][2]);
+            long long t3 = (long long)fma(-qq, p.cc->Mk[3], s[e][3]);
+            // carries: t0, t1, t2 into [0, 2^32) (floor division by 2^32)
+            constexpr long long B32 = 4294967296ll;
+            long long c = t0 >> 32;
+            t0 -= c * B32;
+            t1 += c;
+            c = t1 >> 32;
+            t1 -= c * B32;
+            t2 += c;
+            c = t2 >> 32;
+            t2 -= c * B32;
+            t3 += c;
+            // V = hi 2^64 + lo with lo taken signed, so small |V| loses nothing
+            long long hi = t3 * B32 + t2;
+            unsigned long long ulo = ((unsigned long long)t1 << 32) | (unsigned long long)t0;
+            long long lo = (long long)ulo;
+            if (ulo >> 63) hi += 1;   // lo as signed = ulo - 2^64
+            const double V = fma((double)hi, two32 * two32, (double)lo);
+            const double v = p.alpha * ldexp(V, p.ea[i] + p.eb[j]);
+            double *dst = p.C + i + (int64_t)j * p.ldc;
+            *dst = (p.beta == 0.0) ? v : v + p.beta * *dst;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- host side of the CRT form
+static const int kCrtModuli[OZ_CRT_MAXMOD] = {256, 255, 253, 251, 247, 241, 239, 233,
+                                              229, 227, 223, 217, 211, 199, 197, 193};
+
+static int inv_mod(int a, int m) {   // a^-1 mod m (gcd(a, m) = 1)
+    int t = 0, nt = 1, r = m, nr = ((a % m) + m) % m;
+    while (nr) {
+        const int qd = r / nr;
+        int tmp = t - qd * nt;
+        t = nt;
+        nt = tmp;
+        tmp = r - qd * nr;
+        r = nr;
+        nr = tmp;
+    }
+    return t < 0 ? t + m : t;
+}
+
+void ozaki_crt_constants(int nmod, CrtConstHost *out) {
+    unsigned __int128 M = 1;
+    for (int l = 0; l < nmod; ++l) M *= (unsigned)kCrtModuli[l];
+    out->nmod = nmod;
+    out->log2M = 0.0;
+    for (int l = 0; l < nmod; ++l) out->log2M += std::log2((double)kCrtModuli[l]);
+    for (int l = 0; l < nmod; ++l) {
+        const int m = kCrtModuli[l];
+        const unsigned __int128 Ml = M / (unsigned)m;
+        const int inv = inv_mod((int)(Ml % (unsigned)m), m);
+        const unsigned __int128 W = Ml * (unsigned)inv;   // < M: no reduction needed
+        out->m[l] = m;
+        for (int k = 0; k < 4; ++k) out->w[l][k] = (double)(uint64_t)((W >> (32 * k)) & 0xFFFFFFFFu);
+    }
+    for (int k = 0; k < 4; ++k) out->Mk[k] = (double)(uint64_t)((M >> (32 * k)) & 0xFFFFFFFFu);
+    out->invM = 1.0 / (double)M;
+}
+
+int ozaki_crt_bits(int nmod, int k) {
+    CrtConstHost h;
+    ozaki_crt_constants(nmod, &h);
+    const int lk = (int)std::ceil(std::log2((double)(k > 1 ? k : 1)));
+    int al = (int)std::floor((h.log2M - 2.0 - lk) / 2.0);
+    return al > 56 ? 56 : al;   // residue arithmetic of oz_residue_kernel is exact up to 2^56
+}
+
+// Device copy of the constants per modulus count (built once, synchronously).
+static const CrtConst *crt_device_constants(int nmod, int *status) {
+    static std::mutex mu;
+    static const CrtConst *table[OZ_CRT_MAXMOD + 1] = {};
+    std::lock_guard<std::mutex> lk(mu);
+    if (table[nmod]) return table[nmod];
+    CrtConstHost h;
+    ozaki_crt_constants(nmod, &h);
+    CrtConst c{};
+    c.nmod = nmod;
+    for (int l = 0; l < nmod; ++l) {
+        c.m[l] = h.m[l];
+        c.invm[l] = 1.0 / h.m[l];
+        for (int k = 0; k < 4; ++k) c.w[l][k] = h.w[l][k];
+    }
+    for (int k = 0; k < 4; ++k) c.Mk[k] = h.Mk[k];
+    c.invM = h.invM;
+    void *d = nullptr;
+    hipError_t e = hipMalloc(&d, sizeof(CrtConst));   // kept for the life of the process
+    if (e == hipSuccess) e = hipMemcpy(d, &c, sizeof(CrtConst), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        *status = (int)e;
+        return nullptr;
+    }
+    table[nmod] = static_cast<const CrtConst *>(d);
+    return table[nmod];
+}
+
+size_t ozaki_crt_workspace_bytes(int rows, int k, int nmod) {
+    const size_t rp = (size_t)ozaki_pad_rows(rows), kp = (size_t)ozaki_pad_k(k);
+    return (size_t)nmod * rp * kp + 256 + rp * sizeof(unsigned long long) + rp * sizeof(int) + 256;
+}
+
+size_t ozaki_crt_accum_bytes(int m, int n, int nmod) {
+    const size_t mn = (size_t)ozaki_pad_rows(m) * (size_t)ozaki_pad_rows(n);
+    return mn * 4 + mn * (size_t)nmod + 256;
+}
+
+static int crt_split(const double *X, int64_t rs, int64_t ks, int R, int K, int al, const CrtConst *cc, int nmod,
+                     void *ws, int8_t **res, int **exps, hipStream_t s) {
+    const int Rp = ozaki_pad_rows(R), Kp = ozaki_pad_k(K);
+    unsigned char *base = static_cast<unsigned char *>(ws);
+    *res = reinterpret_cast<int8_t *>(base);
+    size_t off = ((size_t)nmod * Rp * Kp + 255) & ~(size_t)255;
+    unsigned long long *rowmax = reinterpret_cast<unsigned long long *>(base + off);
+    off += (size_t)Rp * sizeof(unsigned long long);
+    *exps = reinterpret_cast<int *>(base + off);
+    hipLaunchKernelGGL(oz_zero_u64_kernel, dim3(grid1(Rp)), dim3(256), 0, s, rowmax, Rp);
+    if (K > 0 && R > 0)
+        hipLaunchKernelGGL(oz_rowmax_kernel, dim3((K + 63) / 64, (R + 63) / 64), dim3(256), 0, s, X, rs, ks, R, K,
+                           rowmax);
+    hipLaunchKernelGGL(oz_residue_kernel, dim3(Kp / 64, Rp / 64), dim3(256), 0, s, X, rs, ks, R, K, Rp, Kp, al,
+                       rowmax, cc, *res, *exps);
+    return (int)hipGetLastError();
+}
+
+int launch_dgemm_ozaki_crt(bool ta, bool tb, int m, int n, int k, double alpha, const double *A, int64_t lda,
+                           const double *B, int64_t ldb, double beta, double *C, int64_t ldc, int nmod, void *wsA,
+                           void *wsB, void *wsW, hipStream_t s) {
+    if (m <= 0 || n <= 0 || k <= 0) return 0;
+    int st = 0;
+    const CrtConst *cc = crt_device_constants(nmod, &st);
+    if (!cc) return st;
+    const int al = ozaki_crt_bits(nmod, k);
+    int8_t *ra, *rb;
+    int *ea, *eb;
+    int e = crt_split(A, ta ? lda : 1, ta ? 1 : lda, m, k, al, cc, nmod, wsA, &ra, &ea, s);
+    if (e) return e;
+    e = crt_split(B, tb ? 1 : ldb, tb ? ldb : 1, n, k, al, cc, nmod, wsB, &rb, &eb, s);
+    if (e) return e;
+    constexpr int WM = 2, WN = 4, TM = 4, TN = 2, KB = 128;
+    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
+    CrtParams p;
+    p.M = m;
+    p.N = n;
+    p.Kp = ozaki_pad_k(k);
+    p.tiles_m = (m + BM - 1) / BM;
+    p.tiles_n = (n + BN - 1) / BN;
+    p.planeA = (int64_t)ozaki_pad_rows(m) * p.Kp;
+    p.planeB = (int64_t)ozaki_pad_rows(n) * p.Kp;
+    p.RA = ra;
+    p.RB = rb;
+    p.ea = ea;
+    p.eb = eb;
+    p.cc = cc;
+    const size_t mn = (size_t)ozaki_pad_rows(m) * (size_t)ozaki_pad_rows(n);
+    p.Wi = static_cast<int *>(wsW);
+    p.Rv = reinterpret_cast<unsigned *>(static_cast<char *>(wsW) + mn * 4);
+    p.alpha = alpha;
+    p.beta = beta;
+    p.C = C;
+    p.ldc = ldc;
+    constexpr size_t lds = 2ull * (BM + BN) * (KB + 16);
+    auto kern = ozaki_crt_kernel<WM, WN, TM, TN, KB>;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t he =
+            hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (he != hipSuccess) return (int)he;
+        attr = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(p.tiles_m * p.tiles_n), dim3(WM * WN * 64), lds, s, p);
+    return (int)hipGetLastError();
+}
+
+int launch_ozaki_crt_split_only(const double *X, int64_t rs, int64_t ks, int R, int K, int nmod, int al,
+                                int8_t *res, int Rp, int Kp, int *exps, unsigned long long *rowmax_ws, hipStream_t s) {
+    if (R <= 0 || K <= 0) return 0;
+    int st = 0;
+    const CrtConst *cc = crt_device_constants(nmod, &st);
+    if (!cc) return st;
+    hipLaunchKernelGGL(oz_zero_u64_kernel, dim3(grid1(Rp)), dim3(256), 0, s, rowmax_ws, Rp);
+    hipLaunchKernelGGL(oz_rowmax_kernel, dim3((K + 63) / 64, (R + 63) / 64), dim3(256), 0, s, X, rs, ks, R, K,
+                       rowmax_ws);
+    hipLaunchKernelGGL(oz_residue_kernel, dim3(Kp / 64, Rp / 64), dim3(256), 0, s, X, rs, ks, R, K, Rp, Kp, al,
+                       rowmax_ws, cc, res, exps);
     return (int)hipGetLastError();
 }
 

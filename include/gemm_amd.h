@@ -125,6 +125,29 @@ int gemm_amd_dgemm_ozaki(char transa, char transb, int m, int n, int k, double a
 int gemm_amd_ozaki_split(const double *x, int64_t row_stride, int64_t k_stride, int rows, int k,
                          int nslices, int8_t *slices, int rows_pad, int k_pad, int *exps, void *stream);
 
+/* Emulated DGEMM on the int8 matrix cores, modular form (no reference
+ * counterpart; Ozaki-Uchino-Imamura "Ozaki scheme II", 2025).  Same contract
+ * as gemm_amd_dgemm.  Rows of op(A) / columns of op(B) are scaled by powers of
+ * two to integers of `bits` = gemm_amd_ozaki_crt_bits(nmoduli, k) bits, reduced
+ * modulo nmoduli (2..16) pairwise coprime moduli <= 256 (256, 255, 253, 251,
+ * 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193), multiplied as
+ * one exact int8 GEMM per modulus, and rebuilt by the Chinese remainder
+ * theorem in exact FP64/int64 limb arithmetic.  nmoduli int8 GEMMs in all;
+ * 16 keeps 55 bits per operand at K = 8192. */
+int gemm_amd_dgemm_ozaki_crt(char transa, char transb, int m, int n, int k, double alpha,
+                             const double *a, int lda, const double *b, int ldb, double beta,
+                             double *c, int ldc, int nmoduli, void *stream);
+/* Bits kept per row/column: the largest b with k*2^(2b) <= M/4 (M = product of
+ * the moduli), at most 56; -1 for invalid arguments. */
+int gemm_amd_ozaki_crt_bits(int nmoduli, int k);
+/* The residue step alone (exposed for byte-exact checks): rows of the view of
+ * x become integers rint(x * 2^(bits - e_r)) (e_r from frexp of the row's max),
+ * whose symmetric residues modulo the first nmoduli moduli are written as
+ * [l][rows_pad][k_pad] int8 planes; exps[r] = e_r - bits. */
+int gemm_amd_ozaki_crt_split(const double *x, int64_t row_stride, int64_t k_stride, int rows, int k,
+                             int nmoduli, int bits, int8_t *residues, int rows_pad, int k_pad, int *exps,
+                             void *stream);
+
 /* SGEMM on the bf16 matrix cores from exact bf16 splits of the FP32 operands
  * (no reference counterpart: the reference's SGEMM is FP32 FMA,
  * src/kernel/gemm_kernel.S:11-23).  Same contract as gemm_amd_sgemm; nprod = 3

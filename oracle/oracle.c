@@ -391,6 +391,36 @@ void oracle_ozaki_split(int rows, int k, const double *x, int64_t row_stride, in
     }
 }
 
+/* Modular (CRT) form of the emulated DGEMM (Ozaki scheme II; no reference
+ * counterpart): row r -> integers rint(x * 2^(bits - e_r)) with e_r from frexp
+ * of the row's max |x| (exact: power-of-two scaling, |value| <= 2^bits <= 2^56
+ * fits int64), then the symmetric residue modulo each modulus by int64
+ * remainder.  A residue of +-m/2 (m even) is stored as the byte of -m/2
+ * (the same residue class). */
+void oracle_ozaki_crt_split(int rows, int k, const double *x, int64_t row_stride, int64_t k_stride, int nmod,
+                            const int *moduli, int bits, int8_t *res, int *exps) {
+    for (int r = 0; r < rows; ++r) {
+        double amax = 0;
+        for (int p = 0; p < k; ++p) {
+            double v = fabs(x[r * row_stride + p * k_stride]);
+            if (v > amax) amax = v;
+        }
+        int e = 0;
+        if (amax > 0) frexp(amax, &e);
+        exps[r] = e - bits;
+        for (int p = 0; p < k; ++p) {
+            const int64_t v = (int64_t)rint(ldexp(x[r * row_stride + p * k_stride], bits - e));
+            for (int l = 0; l < nmod; ++l) {
+                const int64_t m = moduli[l];
+                int64_t q = v % m;                /* in (-m, m) */
+                if (q > m / 2) q -= m;
+                if (q < -(m / 2)) q += m;
+                res[(int64_t)l * rows * k + (int64_t)r * k + p] = (int8_t)(uint8_t)(q & 0xFF);
+            }
+        }
+    }
+}
+
 uint16_t oracle_f32_to_bf16(float x) {
     uint32_t u;
     memcpy(&u, &x, 4);

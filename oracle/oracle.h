@@ -69,6 +69,9 @@ void oracle_err_f32(int64_t n, const float *c, const double *ref_hi, double *out
 /* Ozaki int8 slicing of op(A) rows (or op(B) columns): see gemm_avx512f_amd/csrc/ozaki.hip */
 void oracle_ozaki_split(int rows, int k, const double *x, int64_t row_stride, int64_t k_stride,
                         int nslices, int8_t *slices, int *exps);
+/* emulated DGEMM, modular form: integer scaling and symmetric residues */
+void oracle_ozaki_crt_split(int rows, int k, const double *x, int64_t row_stride, int64_t k_stride, int nmod,
+                            const int *moduli, int bits, int8_t *res /* nmod*rows*k */, int *exps);
 /* bf16 round-to-nearest-even of an FP32 value, and the 2/3-way bf16 split */
 uint16_t oracle_f32_to_bf16(float x);
 float oracle_bf16_to_f32(uint16_t h);

@@ -191,6 +191,29 @@ def ozaki_split(x, nslices, by="rows"):
     return sl, ex
 
 
+CRT_MODULI = (256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193)
+
+
+def ozaki_crt_split(x, nmod, bits, by="rows"):
+    """Symmetric residues of the integer-scaled rows (or columns) of a float64 array modulo the first
+    nmod moduli; returns (res[l, r, k] int8, exps[r] = e_r - bits)."""
+    x = np.asfortranarray(x, dtype=np.float64)
+    if by == "rows":
+        rows, k = x.shape
+        rs, ks = 1, x.shape[0]
+    else:
+        k, rows = x.shape
+        rs, ks = x.shape[0], 1
+    res = np.zeros((nmod, rows, k), dtype=np.int8)
+    ex = np.zeros(rows, dtype=np.int32)
+    mods = np.array(CRT_MODULI, dtype=np.int32)
+    L = lib()
+    L.oracle_ozaki_crt_split.argtypes = [_i, _i, _vp, _i64, _i64, _i, _vp, _i, _vp, _vp]
+    L.oracle_ozaki_crt_split.restype = None
+    L.oracle_ozaki_crt_split(rows, k, _p(x), rs, ks, nmod, _p(mods), bits, _p(res), _p(ex))
+    return res, ex
+
+
 def bf16_split(x, nsplit):
     x = np.ascontiguousarray(x, dtype=np.float32).ravel()
     parts = np.zeros((nsplit, x.size), dtype=np.uint16)

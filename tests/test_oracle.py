@@ -291,3 +291,36 @@ def test_bf16_rounding_known_values_and_split_exactness():
     p3 = O.bf16_split(x, 3)
     r3 = O.bf16_to_f32(p3[0]).astype(np.float64) + O.bf16_to_f32(p3[1]) + O.bf16_to_f32(p3[2])
     assert np.all(np.abs(r3 - x) <= np.abs(x) * 2.0 ** -24)
+
+
+def test_ozaki_crt_split_residues_rebuild_the_scaled_integers():
+    """Residues of the modular form: each is the symmetric residue of the row's scaled integer, and the
+    Chinese remainder theorem over all 16 moduli (exact Python integers) gives that integer back."""
+    from math import prod
+    s = O.Stream(10)
+    x = s.f64(20, 30, r=8)
+    mods = O.CRT_MODULI
+    M = prod(mods)
+    for bits in (20, 48, 56):
+        res, ex = O.ozaki_crt_split(x, 16, bits, by="rows")
+        amax = np.max(np.abs(x), axis=1)
+        assert np.all((amax < 2.0 ** (ex + bits)) & (amax >= 2.0 ** (ex + bits - 1)))
+        for r in range(0, 20, 7):
+            for p in range(0, 30, 11):
+                v = float(x[r, p]) * 2.0 ** (bits - (int(ex[r]) + bits))      # x * 2^(bits - e_r)
+                want = int(np.rint(v))
+                assert abs(want) <= 2 ** bits
+                acc = 0
+                for l, m in enumerate(mods):
+                    q = int(res[l, r, p])
+                    assert -m // 2 <= q <= m // 2 or (m == 256 and q == -128)
+                    assert (want - q) % m == 0
+                    Ml = M // m
+                    acc += q * Ml * pow(Ml, -1, m)
+                acc %= M
+                if acc > M // 2:
+                    acc -= M
+                assert acc == want
+    rc, exc = O.ozaki_crt_split(x, 7, 40, by="cols")
+    rr, exr = O.ozaki_crt_split(np.asfortranarray(x.T), 7, 40, by="rows")
+    assert np.array_equal(rc, rr) and np.array_equal(exc, exr)
