@@ -103,6 +103,87 @@ def gpu_leg(size, steps, warmup, variants=(("crt", 16), ("slices", 8)), seed=0):
     return out
 
 
+def c4_shapes(seed, count=512):
+    """M, N, K log-uniform over [32, 2048] rounded to multiples of 8; transposes drawn per problem."""
+    rng = np.random.default_rng(seed)
+    dims = np.exp(rng.uniform(np.log(32), np.log(2048), size=(count, 3)))
+    dims = np.clip(np.rint(dims / 8) * 8, 32, 2048).astype(int)
+    ts = rng.integers(0, 2, size=(count, 2))
+    return [(int(m), int(n), int(k), "NT"[a], "NT"[b]) for (m, n, k), (a, b) in zip(dims, ts)]
+
+
+def grouped_leg(replays=20):
+    """Batching workloads: 512-problem grouped GEMM (FP32 and bf16 -> FP32) replayed
+    from a captured hipGraph, and a batch of 128 complex-double 3M products of
+    size 256..1024 (one batched call)."""
+    from gemm_avx512f_amd import _lib
+    from gemm_avx512f_amd.device import DeviceMatrix, Event, Stream, fill_uniform
+    from gemm_avx512f_amd.grouped import GROUPED_BF16, GROUPED_F32, GroupedGemm, zgemm3m_batched
+
+    _lib.check(_lib.lib().gemm_amd_init(), "init")
+    out = {}
+    stream = Stream()
+    for name, dtype in (("fp32", GROUPED_F32), ("bf16_fp32acc", GROUPED_BF16)):
+        probs, flops = [], 0.0
+        for i, (m, n, k, ta, tb) in enumerate(c4_shapes(2024)):
+            ar, ac = (m, k) if ta == "N" else (k, m)
+            br, bc = (k, n) if tb == "N" else (n, k)
+            if dtype == GROUPED_F32:
+                a, b = DeviceMatrix(ar, ac, np.float32), DeviceMatrix(br, bc, np.float32)
+                d = fill_uniform(a, seed=i)
+                fill_uniform(b, seed=i, first_draw=d)
+            else:
+                # bf16 operands: bit patterns of uniform[-1,1) values truncated to bf16, built on the host
+                rng = np.random.default_rng(i)
+                a = DeviceMatrix.from_host(np.asfortranarray(
+                    (rng.uniform(-1, 1, (ar, ac)).astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)))
+                b = DeviceMatrix.from_host(np.asfortranarray(
+                    (rng.uniform(-1, 1, (br, bc)).astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)))
+            c = DeviceMatrix(m, n, np.float32)
+            probs.append(dict(transa=ta, transb=tb, m=m, n=n, k=k, alpha=1.0, beta=0.0, a=a, b=b, c=c))
+            flops += 2.0 * m * n * k
+        g = GroupedGemm(dtype, probs)
+        g.capture(stream)
+        for _ in range(3):
+            g.replay(stream)
+        stream.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record(stream)
+        for _ in range(replays):
+            g.replay(stream)
+        e1.record(stream)
+        ms = e0.elapsed_ms(e1) / replays
+        out[name] = {"problems": len(probs), "gflop": round(flops / 1e9, 2), "ms_per_replay": round(ms, 4),
+                     "tflops": round(flops / (ms / 1e3) / 1e12, 2), "tiles": g.tiles}
+        g.destroy()
+        del probs
+    rng = np.random.default_rng(128)
+    zp, zflops = [], 0.0
+    for i in range(128):
+        m, n, k = (int(x) for x in rng.integers(256, 1025, size=3))
+        fa, fb = DeviceMatrix(2 * m, k, np.float64), DeviceMatrix(2 * k, n, np.float64)
+        d = fill_uniform(fa, seed=9000 + i)
+        fill_uniform(fb, seed=9000 + i, first_draw=d)
+        za, zb = (DeviceMatrix.__new__(DeviceMatrix) for _ in range(2))
+        for z, src, r, cl in ((za, fa, m, k), (zb, fb, k, n)):
+            z.rows, z.cols, z.dtype, z.ld, z.buf = r, cl, np.dtype(np.complex128), r, src.buf
+        zp.append(dict(transa="N", transb="N", m=m, n=n, k=k, alpha=1.0, beta=0.0, a=za, b=zb,
+                       c=DeviceMatrix(m, n, np.complex128)))
+        zflops += 8.0 * m * n * k
+    zgemm3m_batched(zp, stream)          # warm-up (workspace growth)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        zgemm3m_batched(zp, stream)
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    out["zgemm3m_batch"] = {"problems": 128, "ms_per_call": round(ms, 3),
+                            "tflops_complex_equiv": round(zflops / (ms / 1e3) / 1e12, 2),
+                            "tflops_real_executed": round(0.75 * zflops / (ms / 1e3) / 1e12, 2),
+                            "note": "host-timed (each call builds its plan and synchronises)"}
+    return out
+
+
 def cpu_leg(size, reps=3):
     """Reference dgemm_ (oracle/_ref/DGEMM.so, -O2 -mavx512f -fopenmp) on this host."""
     from oracle import oracle as O
@@ -142,6 +223,7 @@ def main():
     ap.add_argument("--nslices", type=int, default=8, help="slices per operand of the slice-form emulated DGEMM")
     ap.add_argument("--nmoduli", type=int, default=16, help="moduli of the modular-form emulated DGEMM")
     ap.add_argument("--cpu-size", type=int, default=None, help="CPU leg problem size (default: --size)")
+    ap.add_argument("--no-grouped", action="store_true", help="skip the grouped-GEMM / zgemm3m batch leg")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB first (a known-size kernel that "
                          "calibrates the HBM byte counters when the run is profiled)")
@@ -214,6 +296,8 @@ def main():
         ok = [v for v in variants if v["fp64_level"]] or variants
         result["emulated"] = max(ok, key=lambda v: v["tflops_fp64_equiv"])
         result["emulated_variants"] = variants
+    if not args.no_grouped:
+        result["batching"] = grouped_leg()
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_leg(args.cpu_size or size)

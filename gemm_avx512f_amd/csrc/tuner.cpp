@@ -48,7 +48,8 @@ bool tune_valid(int op, int m, int n, int k, const GemmConfig &c) {
 // when even 64x64 tiles leave CUs idle and K is deep, split K so that about
 // two workgroups per CU exist, keeping at least 256 of K per partition.
 // Measured on MI355X at 4096^3..8192^3: FP64 is fastest on 128x64 tiles (two
-// waves per SIMD), FP32 on 64x64 tiles (four waves per SIMD).  The complex 3M
+// waves per SIMD), FP32 on 256x128 then 128x128 tiles (64x64 per wave keeps
+// the LDS read traffic per matrix instruction low).  The complex 3M
 // products use 128x128 tiles when those fill the chip and split K at most 8
 // ways.
 GemmConfig heuristic_gemm_config(int dtype, int m, int n, int k, int cus) {
@@ -57,6 +58,17 @@ GemmConfig heuristic_gemm_config(int dtype, int m, int n, int k, int cus) {
     const long t128x64 = (long)ceil_div(m, 128) * ceil_div(n, 64);
     const long t64 = (long)ceil_div(m, 64) * ceil_div(n, 64);
     if (dtype == DT_C32 || dtype == DT_C64) {
+        // the three real products use the real dtype's tile rule below
+        const bool f64 = dtype == DT_C64;
+        const long t256x128c = (long)ceil_div(m, 256) * ceil_div(n, 128);
+        if (f64 && t128x64 >= cus) {
+            c.tile = TILE_128x64;
+            return c;
+        }
+        if (!f64 && t256x128c >= cus) {
+            c.tile = TILE_LARGE;
+            return c;
+        }
         if (t128 >= cus) return c;   // TILE_128x128
         c.tile = TILE_64x64;
         if (t64 < cus && k >= 512) {
@@ -69,8 +81,13 @@ GemmConfig heuristic_gemm_config(int dtype, int m, int n, int k, int cus) {
         return c;
     }
     const bool fp64 = dtype == DT_F64;
+    const long t256x128 = (long)ceil_div(m, 256) * ceil_div(n, 128);
     if (fp64 && t128x64 >= cus) {
         c.tile = TILE_128x64;
+    } else if (!fp64 && t256x128 >= cus) {
+        c.tile = TILE_LARGE;   // FP32: 256x128 tiles of eight 64x64 waves, once they fill the chip
+    } else if (!fp64 && t128 >= cus) {
+        c.tile = TILE_128x128;
     } else if (t64 >= cus) {
         c.tile = TILE_64x64;
     } else {

@@ -127,10 +127,13 @@ def sweep(op, ta, tb, m, n, k, pad=0):
     return np.array(ms[:], dtype=np.float32)
 
 
-def sweep_shapes(shapes, log=None, every=100):
-    out = np.zeros((len(shapes), NCFG + 1), dtype=np.float32)
+def sweep_shapes(shapes, log=None, every=100, ops=None):
+    """Sweep every shape (or only those whose op is in `ops`; the others are left at -2)."""
+    out = np.full((len(shapes), NCFG + 1), -2.0, dtype=np.float32)
     t0 = time.time()
     for i, s in enumerate(shapes):
+        if ops is not None and int(s["op"]) not in ops:
+            continue
         out[i] = sweep(int(s["op"]), bool(s["ta"]), bool(s["tb"]), int(s["m"]), int(s["n"]), int(s["k"]),
                        int(s["pad"]))
         if log is not None and (i + 1) % every == 0:
@@ -230,6 +233,20 @@ def write_weights_inc(params, path=WEIGHTS_INC, note=""):
         f.write("\n".join(lines))
 
 
+def regret_report(shapes, ms, choice):
+    """Median / mean / 90th-percentile regret of `choice` and of the heuristic, overall and per op."""
+    r_pick = regret(ms, np.asarray(choice))
+    r_heur = regret(ms, np.full(len(ms), NCFG))
+    out = {"shapes": int(len(ms))}
+    groups = [("all", np.ones(len(ms), dtype=bool))] + [(OP_NAMES[o], shapes["op"] == o) for o in OP_NAMES]
+    for name, sel in groups:
+        out[name] = {key: {"median": float(np.median(r[sel])), "mean": float(np.mean(r[sel])),
+                           "p90": float(np.percentile(r[sel], 90)),
+                           "exact_best": float(np.mean(r[sel] <= 0.0))}
+                     for key, r in (("picker", r_pick), ("heuristic", r_heur))}
+    return out
+
+
 def load_dataset(paths):
     shapes, ms = [], []
     for p in paths:
@@ -249,6 +266,7 @@ def _main(argv):
     s.add_argument("--start", type=int, default=0)
     s.add_argument("--count", type=int, required=True)
     s.add_argument("--out", required=True)
+    s.add_argument("--ops", type=int, nargs="+", default=None, help="sweep only these ops (others stay -2)")
     t = sub.add_parser("train")
     t.add_argument("--data", nargs="+", required=True)
     t.add_argument("--out", required=True)
@@ -256,13 +274,27 @@ def _main(argv):
     t.add_argument("--batch", type=int, default=256)
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--seed", type=int, default=0)
+    ev = sub.add_parser("evaluate")
+    ev.add_argument("--weights", required=True)
+    ev.add_argument("--data", nargs="+", required=True)
+    ev.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     log = lambda msg: print(msg, flush=True)  # noqa: E731
+    if a.cmd == "evaluate":
+        import json
+        shapes, ms = load_dataset(a.data)
+        picker_load(np.load(a.weights)["params"])
+        stats = regret_report(shapes, ms, picker_choose(shapes))
+        log(json.dumps(stats))
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump(stats, f, indent=1)
+        return 0
     if a.cmd == "sweep":
         total = a.total or (a.start + a.count)
         shapes = sample_shapes(a.seed, total)[a.start:a.start + a.count]
         log("sweeping %d shapes (seed %d, %d..%d)" % (len(shapes), a.seed, a.start, a.start + len(shapes)))
-        ms = sweep_shapes(shapes, log=log)
+        ms = sweep_shapes(shapes, log=log, ops=set(a.ops) if a.ops else None)
         np.savez_compressed(a.out, shapes=shapes, ms=ms, seed=a.seed, start=a.start)
         log("wrote " + a.out)
     else:
