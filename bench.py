@@ -103,6 +103,34 @@ def gpu_leg(size, steps, warmup, variants=(("crt", 16), ("slices", 8)), seed=0):
     return out
 
 
+def sgemm_leg(size, steps, warmup, seed=0):
+    """SGEMM size^3 NN: native FP32 matrix cores vs the bf16x3 / bf16x6 split
+    forms on the same FP32 inputs, each with its sampled error."""
+    from gemm_avx512f_amd import blas
+    from gemm_avx512f_amd.device import DeviceMatrix, Stream, fill_uniform
+
+    m = n = k = size
+    stream = Stream()
+    A, B, C = DeviceMatrix(m, k, np.float32), DeviceMatrix(k, n, np.float32), DeviceMatrix(m, n, np.float32)
+    d = fill_uniform(A, seed, 0, stream=stream)
+    fill_uniform(B, seed, d, stream=stream)
+    a, b = A.to_host().astype(np.float64), B.to_host().astype(np.float64)
+    rng = np.random.default_rng(54321)
+    rows, cols = rng.integers(0, m, 256), rng.integers(0, n, 256)
+    flops = 2.0 * m * n * k
+    out = {"workload": "SGEMM %d^3 NN, alpha=1, beta=0, uniform[-1,1) FP32" % size}
+    runs = (("native_fp32", lambda: blas.sgemm("N", "N", m, n, k, 1.0, A, B, 0.0, C, stream=stream), 157.3),
+            ("bf16x3", lambda: blas.sgemm_bf16x("N", "N", m, n, k, 1.0, A, B, 0.0, C, nprod=3, stream=stream), 157.3),
+            ("bf16x6", lambda: blas.sgemm_bf16x("N", "N", m, n, k, 1.0, A, B, 0.0, C, nprod=6, stream=stream), 157.3))
+    for name, fn, peak in runs:
+        ms, _ = _time(fn, stream, steps, warmup)
+        tf = flops * steps / (ms / 1e3) / 1e12
+        err = _sampled_error(a, b, C.to_host().astype(np.float64), rows, cols)
+        out[name] = {"tflops": round(tf, 2), "pct_fp32_peak": round(100.0 * tf / peak, 2),
+                     "ms_per_step": round(ms / steps, 4), "max_rel": err[0], "rms_rel": err[1]}
+    return out
+
+
 def c4_shapes(seed, count=512):
     """M, N, K log-uniform over [32, 2048] rounded to multiples of 8; transposes drawn per problem."""
     rng = np.random.default_rng(seed)
@@ -224,6 +252,7 @@ def main():
     ap.add_argument("--nmoduli", type=int, default=16, help="moduli of the modular-form emulated DGEMM")
     ap.add_argument("--cpu-size", type=int, default=None, help="CPU leg problem size (default: --size)")
     ap.add_argument("--no-grouped", action="store_true", help="skip the grouped-GEMM / zgemm3m batch leg")
+    ap.add_argument("--no-sgemm", action="store_true", help="skip the SGEMM / bf16x split leg")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB first (a known-size kernel that "
                          "calibrates the HBM byte counters when the run is profiled)")
@@ -296,6 +325,8 @@ def main():
         ok = [v for v in variants if v["fp64_level"]] or variants
         result["emulated"] = max(ok, key=lambda v: v["tflops_fp64_equiv"])
         result["emulated_variants"] = variants
+    if not args.no_sgemm:
+        result["sgemm"] = sgemm_leg(size, args.steps, args.warmup)
     if not args.no_grouped:
         result["batching"] = grouped_leg()
     if not args.no_cpu:

@@ -10,8 +10,9 @@
 // so big tiles are spread over all CUs first and no atomic counter has to be
 // reset between launches (the launch is replayable from a captured hipGraph).
 // A tile finds its problem by binary search over tile_base and runs the same
-// LDS-staged matrix-core tile as the single-problem kernel, with per-element
-// range checks (problem sizes are arbitrary multiples of 1).
+// LDS-staged matrix-core tile as the single-problem kernel, with range checks
+// at the problem's edges (sizes are arbitrary) and 16-byte vector loads for
+// problems whose operands are aligned for them.
 #include "common.hpp"
 #include "gemm_tile.hpp"
 #include "grouped.hpp"
@@ -22,7 +23,7 @@
 namespace gemm_amd {
 
 struct GroupDesc {
-    int m, n, k, ta, tb, tiles_m, tiles_n, pad;
+    int m, n, k, ta, tb, tiles_m, tiles_n, vec;   // vec: A, B 16-byte aligned with ld a multiple of 16 bytes
     int64_t lda, ldb, ldc, tile_base;
     const void *A, *B;
     void *C;
@@ -30,12 +31,12 @@ struct GroupDesc {
 };
 
 // One BM x BN output tile of problem d; TI = operand element type, TO = C's.
-template <typename TI, typename TO, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN>
-__device__ void grouped_tile(const GroupDesc &d, int tm, int tn, unsigned char *smem_raw) {
+template <typename TI, typename TO, bool TA, bool TB, bool VEC, int BM, int BN, int BK, int WM, int WN>
+__device__ void grouped_tile(const GroupDesc &d, int tm, int tn, unsigned char *smem_raw) {   // d: in global memory
     constexpr int NT = WM * WN * 64;
     constexpr int WTM = BM / WM, WTN = BN / WN;
-    using LA = TileLoader<TI, BM, BK, NT, TA, false>;
-    using LB = TileLoader<TI, BN, BK, NT, !TB, false>;
+    using LA = TileLoader<TI, BM, BK, NT, TA, VEC>;
+    using LB = TileLoader<TI, BN, BK, NT, !TB, VEC>;
     using Wave = typename WaveSel<TI, WTM, WTN, BK>::type;
     constexpr int PITCH = LA::PITCH;
     constexpr int STAGE = (BM + BN) * PITCH;
@@ -99,14 +100,18 @@ __global__ __launch_bounds__(WM *WN * 64) void grouped_gemm_kernel(const GroupDe
             if (descs[mid].tile_base <= t) lo = mid;
             else hi = mid - 1;
         }
-        const GroupDesc d = descs[lo];
+        const GroupDesc &d = descs[lo];   // read in place (uniform scalar loads), never copied to private memory
         const int local = (int)(t - d.tile_base);
         const int tm = local % d.tiles_m, tn = local / d.tiles_m;
-        switch (d.ta * 2 + d.tb) {
-        case 0: grouped_tile<TI, TO, false, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
-        case 1: grouped_tile<TI, TO, false, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
-        case 2: grouped_tile<TI, TO, true, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
-        default: grouped_tile<TI, TO, true, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        switch (d.vec * 4 + d.ta * 2 + d.tb) {
+        case 0: grouped_tile<TI, TO, false, false, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        case 1: grouped_tile<TI, TO, false, true, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        case 2: grouped_tile<TI, TO, true, false, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        case 3: grouped_tile<TI, TO, true, true, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        case 4: grouped_tile<TI, TO, false, false, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        case 5: grouped_tile<TI, TO, false, true, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        case 6: grouped_tile<TI, TO, true, false, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        default: grouped_tile<TI, TO, true, true, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
         }
         __syncthreads();   // the next tile reuses the LDS stages
     }
@@ -172,7 +177,12 @@ void *grouped_plan_create(int dtype, int count, const char *ta, const char *tb, 
         d.tb = !(tb[p] == 'N' || tb[p] == 'n');
         d.tiles_m = (m[p] + BM - 1) / BM;
         d.tiles_n = (n[p] + BN - 1) / BN;
-        d.pad = 0;
+        {
+            const int eb = dtype == GROUPED_F64 ? 8 : (dtype == GROUPED_F32 ? 4 : 2);
+            const int vw = 16 / eb;
+            d.vec = ((uintptr_t)A[p] % 16 == 0) && ((uintptr_t)B[p] % 16 == 0) && (lda[p] % vw == 0) &&
+                    (ldb[p] % vw == 0);
+        }
         d.lda = lda[p];
         d.ldb = ldb[p];
         d.ldc = ldc[p];

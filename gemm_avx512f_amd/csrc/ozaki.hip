@@ -491,6 +491,7 @@ struct CrtParams {
     const int *ea, *eb;
     const CrtConst *cc;
     int *Wi;                      // int32 staging of the accumulators (per wave, fragment order)
+    int *Wc;                      // running residues when K spans several exact chunks, same order
     unsigned *Rv;                 // product residues, 4 per word, per wave [l][idx/4][lane]
     double alpha, beta;
     double *C;
@@ -527,6 +528,7 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
     const int total = nmod * nkt;
     const int64_t wv = (int64_t)(tm * p.tiles_n + tn) * (WM * WN) + wave;
     int *Wi = p.Wi + wv * (NACC * 64);
+    int *Wc = p.Wc + wv * (NACC * 64);
     unsigned *Rv = p.Rv + wv * ((int64_t)nmod * (NACC / 4) * 64);
 
     i4_t ra[PA], rb[PB];
@@ -573,7 +575,7 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
     for (int l = 0; l < nmod; ++l) {
         const int m = p.cc->m[l];
         const double im = p.cc->invm[l];
-        bool carry = false;   // Wi holds a reduced partial sum of this modulus
+        const bool multi = nkt > CHUNK;   // K longer than one exact int32 chunk
         for (int c0 = 0; c0 < nkt; c0 += CHUNK) {
             const int c1 = min(nkt, c0 + CHUNK);
 #pragma unroll
@@ -614,22 +616,22 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
 #pragma unroll
                 for (int b = 0; b < TN; ++b)
 #pragma unroll
-                    for (int e = 0; e < 16; ++e) {
-                        int *dst = wb + ((a * TN + b) * 16 + e) * 64;
-                        *dst = carry ? acc[a][b][e] + *dst : acc[a][b][e];
-                    }
-            if (c1 < nkt) {
-                // K longer than one exact chunk: fold the partial sum to a residue
-                for (int idx = 0; idx < NACC; ++idx) Wi[idx * 64 + lane] = crt_mod(Wi[idx * 64 + lane], m, im);
-                carry = true;
+                    for (int e = 0; e < 16; ++e) wb[((a * TN + b) * 16 + e) * 64] = acc[a][b][e];
+            if (multi) {
+                // fold this chunk into the running residue (a memory loop)
+                for (int idx = 0; idx < NACC; ++idx) {
+                    const int prev = c0 == 0 ? 0 : Wc[idx * 64 + lane];
+                    Wc[idx * 64 + lane] = crt_mod(prev + Wi[idx * 64 + lane], m, im);
+                }
             }
         }
+        const int *src = multi ? Wc : Wi;
         // residues of this modulus, four per word, in accumulator order
         for (int w4 = 0; w4 < NACC / 4; ++w4) {
             unsigned word = 0;
 #pragma unroll
             for (int e = 0; e < 4; ++e)
-                word |= (unsigned)(crt_mod(Wi[(w4 * 4 + e) * 64 + lane], m, im) & 0xFF) << (8 * e);
+                word |= (unsigned)(crt_mod(src[(w4 * 4 + e) * 64 + lane], m, im) & 0xFF) << (8 * e);
             Rv[((int64_t)l * (NACC / 4) + w4) * 64 + lane] = word;
         }
     }
@@ -770,7 +772,7 @@ size_t ozaki_crt_workspace_bytes(int rows, int k, int nmod) {
 
 size_t ozaki_crt_accum_bytes(int m, int n, int nmod) {
     const size_t mn = (size_t)ozaki_pad_rows(m) * (size_t)ozaki_pad_rows(n);
-    return mn * 4 + mn * (size_t)nmod + 256;
+    return mn * 8 + mn * (size_t)nmod + 256;
 }
 
 static int crt_split(const double *X, int64_t rs, int64_t ks, int R, int K, int al, const CrtConst *cc, int nmod,
@@ -791,6 +793,27 @@ static int crt_split(const double *X, int64_t rs, int64_t ks, int R, int K, int 
     return (int)hipGetLastError();
 }
 
+// Tile layouts of the CRT kernel share one workspace layout: the int32 staging
+// and the residues are per output element (tile area), whatever the wave split.
+template <int WM, int WN, int TM, int TN, int KB>
+static int launch_crt_variant(CrtParams &p, hipStream_t s) {
+    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
+    static_assert(BM == 256 && BN == 256, "rows are padded for 256-row tiles");
+    p.tiles_m = (p.M + BM - 1) / BM;
+    p.tiles_n = (p.N + BN - 1) / BN;
+    constexpr size_t lds = 2ull * (BM + BN) * (KB + 16);
+    auto kern = ozaki_crt_kernel<WM, WN, TM, TN, KB>;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t he =
+            hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (he != hipSuccess) return (int)he;
+        attr = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(p.tiles_m * p.tiles_n), dim3(WM * WN * 64), lds, s, p);
+    return (int)hipGetLastError();
+}
+
 int launch_dgemm_ozaki_crt(bool ta, bool tb, int m, int n, int k, double alpha, const double *A, int64_t lda,
                            const double *B, int64_t ldb, double beta, double *C, int64_t ldc, int nmod, void *wsA,
                            void *wsB, void *wsW, hipStream_t s) {
@@ -805,14 +828,10 @@ int launch_dgemm_ozaki_crt(bool ta, bool tb, int m, int n, int k, double alpha, 
     if (e) return e;
     e = crt_split(B, tb ? 1 : ldb, tb ? ldb : 1, n, k, al, cc, nmod, wsB, &rb, &eb, s);
     if (e) return e;
-    constexpr int WM = 2, WN = 4, TM = 4, TN = 2, KB = 128;
-    constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
     CrtParams p;
     p.M = m;
     p.N = n;
     p.Kp = ozaki_pad_k(k);
-    p.tiles_m = (m + BM - 1) / BM;
-    p.tiles_n = (n + BN - 1) / BN;
     p.planeA = (int64_t)ozaki_pad_rows(m) * p.Kp;
     p.planeB = (int64_t)ozaki_pad_rows(n) * p.Kp;
     p.RA = ra;
@@ -822,22 +841,16 @@ int launch_dgemm_ozaki_crt(bool ta, bool tb, int m, int n, int k, double alpha, 
     p.cc = cc;
     const size_t mn = (size_t)ozaki_pad_rows(m) * (size_t)ozaki_pad_rows(n);
     p.Wi = static_cast<int *>(wsW);
-    p.Rv = reinterpret_cast<unsigned *>(static_cast<char *>(wsW) + mn * 4);
+    p.Wc = p.Wi + mn;
+    p.Rv = reinterpret_cast<unsigned *>(static_cast<char *>(wsW) + mn * 8);
     p.alpha = alpha;
     p.beta = beta;
     p.C = C;
     p.ldc = ldc;
-    constexpr size_t lds = 2ull * (BM + BN) * (KB + 16);
-    auto kern = ozaki_crt_kernel<WM, WN, TM, TN, KB>;
-    static bool attr = false;
-    if (!attr) {
-        hipError_t he =
-            hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (he != hipSuccess) return (int)he;
-        attr = true;
-    }
-    hipLaunchKernelGGL(kern, dim3(p.tiles_m * p.tiles_n), dim3(WM * WN * 64), lds, s, p);
-    return (int)hipGetLastError();
+    static const char *var = getenv("GEMM_AMD_CRT_VARIANT");
+    const int v = var ? atoi(var) : 0;
+    if (v == 2) return launch_crt_variant<2, 4, 4, 2, 64>(p, s);    // 256x256 tile, 8 waves, 64-deep stages
+    return launch_crt_variant<2, 4, 4, 2, 128>(p, s);               // 256x256 tile, 8 waves of 128x64
 }
 
 int launch_ozaki_crt_split_only(const double *X, int64_t rs, int64_t ks, int R, int K, int nmod, int al,

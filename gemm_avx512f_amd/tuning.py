@@ -247,6 +247,18 @@ def regret_report(shapes, ms, choice):
     return out
 
 
+def merge_sweeps(old_path, new_path, out_path):
+    """Replace the rows of an older sweep that a newer partial sweep (--ops) measured again."""
+    old, new = np.load(old_path, allow_pickle=False), np.load(new_path, allow_pickle=False)
+    if not np.array_equal(old["shapes"], new["shapes"]):
+        raise ValueError("the two sweeps cover different shape lists")
+    ms = old["ms"].copy()
+    fresh = new["ms"][:, 0] != -2.0
+    ms[fresh] = new["ms"][fresh]
+    np.savez_compressed(out_path, shapes=old["shapes"], ms=ms, seed=old["seed"], start=old["start"])
+    return int(fresh.sum())
+
+
 def load_dataset(paths):
     shapes, ms = [], []
     for p in paths:
