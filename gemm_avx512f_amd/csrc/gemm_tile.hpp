@@ -26,19 +26,20 @@ struct GemmParams {
 // k contiguous (element (r,k) at base[k + r*ld]); otherwise rows contiguous
 // (element (r,k) at base[r + k*ld]).  Elements outside (rlim, klim) read 0.
 //
-// Rows-contiguous FP32 operands are gathered "k-quad" wise: a thread reads
-// the 4 consecutive k of one row with 4 scalar loads (neighbouring lanes read
-// neighbouring rows, so every load is one coalesced 256-byte segment per
-// wave) and writes them as one 16-byte LDS vector.  Transposing in the other
-// direction (a 16-byte global load of 4 rows, scattered into 4 LDS rows) puts
-// 32 lanes of a wave on 2 LDS banks with a 36-float row pitch.
+// Rows-contiguous FP32/FP64 operands are gathered k-wise: a thread reads the
+// 4 (FP32) or 2 (FP64) consecutive k of one row with scalar loads
+// (neighbouring lanes read neighbouring rows, so every load is one coalesced
+// 256/512-byte segment per wave) and writes them as one 16-byte LDS vector.
+// Transposing in the other direction (a 16-byte global load of several rows,
+// scattered into as many LDS rows) puts 32 lanes of a wave on 2 LDS banks
+// with a 36-float row pitch, and 16 lanes on 4 banks with an 18-double one.
 template <typename T, int ROWS, int BK, int NT, bool KCONTIG, bool VEC>
 struct TileLoader {
     using V = typename Vec16<T>::t;
     static constexpr int VW = Vec16<T>::N;
     static constexpr int PITCH = BK + VW;
     static constexpr int NVEC = ROWS * BK / VW;
-    static constexpr bool KQUAD = !KCONTIG && sizeof(T) == 4;
+    static constexpr bool KGATHER = !KCONTIG && sizeof(T) >= 4;
     static_assert(NVEC % NT == 0, "tile must split evenly over the workgroup");
     static constexpr int PER = NVEC / NT;
     V reg[PER];
@@ -47,7 +48,7 @@ struct TileLoader {
         if (KCONTIG) {
             r = v / (BK / VW);
             k = (v % (BK / VW)) * VW;
-        } else if (KQUAD) {
+        } else if (KGATHER) {
             r = v % ROWS;
             k = (v / ROWS) * VW;
         } else {
@@ -63,7 +64,7 @@ struct TileLoader {
             int r, k;
             coords(tid + i * NT, r, k);
             const T *src = KCONTIG ? base + (int64_t)r * ld + k : base + r + (int64_t)k * ld;
-            if (KQUAD) {
+            if (KGATHER) {
 #pragma unroll
                 for (int e = 0; e < VW; ++e) reg[i][e] = src[(int64_t)e * ld];
             } else {
@@ -79,7 +80,7 @@ struct TileLoader {
             int r, k;
             coords(tid + i * NT, r, k);
             const T *src = KCONTIG ? base + (int64_t)r * ld + k : base + r + (int64_t)k * ld;
-            if (KQUAD) {
+            if (KGATHER) {
                 V t;
 #pragma unroll
                 for (int e = 0; e < VW; ++e) t[e] = (r < rlim && k + e < klim) ? src[(int64_t)e * ld] : T(0);
@@ -106,7 +107,7 @@ struct TileLoader {
         for (int i = 0; i < PER; ++i) {
             int r, k;
             coords(tid + i * NT, r, k);
-            if (KCONTIG || KQUAD) {
+            if (KCONTIG || KGATHER) {
                 *reinterpret_cast<V *>(lds + r * PITCH + k) = reg[i];
             } else {
 #pragma unroll

@@ -290,8 +290,28 @@ def _main(argv):
     ev.add_argument("--weights", required=True)
     ev.add_argument("--data", nargs="+", required=True)
     ev.add_argument("--out", default=None)
+    one = sub.add_parser("shape", help="time every candidate of one shape and print TFLOP/s per candidate")
+    one.add_argument("--op", type=int, required=True)
+    one.add_argument("--ta", type=int, default=0)
+    one.add_argument("--tb", type=int, default=0)
+    one.add_argument("--m", type=int, required=True)
+    one.add_argument("--n", type=int, required=True)
+    one.add_argument("--k", type=int, required=True)
+    one.add_argument("--pad", type=int, default=0)
     a = ap.parse_args(argv)
     log = lambda msg: print(msg, flush=True)  # noqa: E731
+    if a.cmd == "shape":
+        ms = sweep(a.op, bool(a.ta), bool(a.tb), a.m, a.n, a.k, a.pad)
+        fl = 2.0 * a.m * a.n * a.k * (3 if a.op == OP_CGEMM3M else 1)
+        cands = candidates()
+        cells = ["%s tile %d split %d: %.1f" % ("*" if c == int(np.argmin(np.where(ms[:NCFG] > 0, ms[:NCFG], np.inf)))
+                                                else " ", cands[c][0], cands[c][1], fl / (ms[c] * 1e-3) / 1e12)
+                 for c in range(NCFG) if ms[c] > 0]
+        log("%s ta=%d tb=%d %dx%dx%d pad %d (TFLOP/s; * = fastest; heuristic's choice %.1f):" % (
+            OP_NAMES[a.op], a.ta, a.tb, a.m, a.n, a.k, a.pad, fl / (ms[NCFG] * 1e-3) / 1e12))
+        for c in cells:
+            log("  " + c)
+        return 0
     if a.cmd == "evaluate":
         import json
         shapes, ms = load_dataset(a.data)
