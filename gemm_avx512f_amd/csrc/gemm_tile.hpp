@@ -45,6 +45,7 @@ struct TileLoader {
     static_assert(NVEC % NT == 0, "tile must split evenly over the workgroup");
     static constexpr int PER = NVEC / NT;
     V reg[PER];
+    bool vec = true;   // run-time veto of the vector path (VEC must also be set)
 
     __device__ __forceinline__ static void coords(int v, int &r, int &k) {
         if (KCONTIG) {
@@ -90,7 +91,7 @@ struct TileLoader {
                 continue;
             }
             const bool full = KCONTIG ? (r < rlim && k + VW <= klim) : (r + VW <= rlim && k < klim);
-            if (VEC && full) {
+            if (VEC && vec && full) {
                 reg[i] = *reinterpret_cast<const V *>(src);
             } else {
                 V t;

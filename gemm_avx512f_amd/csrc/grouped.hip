@@ -31,12 +31,14 @@ struct GroupDesc {
 };
 
 // One BM x BN output tile of problem d; TI = operand element type, TO = C's.
-template <typename TI, typename TO, bool TA, bool TB, bool VEC, int BM, int BN, int BK, int WM, int WN>
+template <typename TI, typename TO, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN>
 __device__ void grouped_tile(const GroupDesc &d, int tm, int tn, unsigned char *smem_raw) {   // d: in global memory
     constexpr int NT = WM * WN * 64;
     constexpr int WTM = BM / WM, WTN = BN / WN;
-    using LA = TileLoader<TI, BM, BK, NT, TA, VEC>;
-    using LB = TileLoader<TI, BN, BK, NT, !TB, VEC>;
+    // vector loads where the problem's operands allow them (a run-time choice:
+    // eight compile-time variants of the tile would crowd the register file)
+    using LA = TileLoader<TI, BM, BK, NT, TA, true>;
+    using LB = TileLoader<TI, BN, BK, NT, !TB, true>;
     using Wave = typename WaveSel<TI, WTM, WTN, BK>::type;
     constexpr int PITCH = LA::PITCH;
     constexpr int STAGE = (BM + BN) * PITCH;
@@ -53,6 +55,7 @@ __device__ void grouped_tile(const GroupDesc &d, int tm, int tn, unsigned char *
     const int64_t bstep = TB ? (int64_t)BK * d.ldb : BK;
     LA la;
     LB lb;
+    la.vec = lb.vec = d.vec != 0;
     Wave w;
     w.init(lane, wm0, wn0);
     const int nk = (d.k + BK - 1) / BK;
@@ -103,15 +106,11 @@ __global__ __launch_bounds__(WM *WN * 64) void grouped_gemm_kernel(const GroupDe
         const GroupDesc &d = descs[lo];   // read in place (uniform scalar loads), never copied to private memory
         const int local = (int)(t - d.tile_base);
         const int tm = local % d.tiles_m, tn = local / d.tiles_m;
-        switch (d.vec * 4 + d.ta * 2 + d.tb) {
-        case 0: grouped_tile<TI, TO, false, false, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
-        case 1: grouped_tile<TI, TO, false, true, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
-        case 2: grouped_tile<TI, TO, true, false, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
-        case 3: grouped_tile<TI, TO, true, true, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
-        case 4: grouped_tile<TI, TO, false, false, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
-        case 5: grouped_tile<TI, TO, false, true, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
-        case 6: grouped_tile<TI, TO, true, false, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
-        default: grouped_tile<TI, TO, true, true, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        switch (d.ta * 2 + d.tb) {
+        case 0: grouped_tile<TI, TO, false, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        case 1: grouped_tile<TI, TO, false, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        case 2: grouped_tile<TI, TO, true, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        default: grouped_tile<TI, TO, true, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
         }
         __syncthreads();   // the next tile reuses the LDS stages
     }
@@ -129,12 +128,18 @@ struct GroupedPlan {
     hipGraphExec_t exec = nullptr;
 };
 
-// Tile configuration per dtype: 64x64 tiles of 4 waves (problems run 32..2048).
+// Tile configuration per dtype: 64x64 tiles of four 32x32 waves.  The grouped
+// problems (tens to a few thousand per dimension) leave large tiles short of
+// work: 128x128 tiles measured 26 / 63 TFLOP/s (FP32 / bf16) on 512 problems
+// of 32..2048 against 44 / 117 for these 64x64 ones.  bf16 takes 64-deep slabs
+// (77 TFLOP/s with 32-deep ones) so a stage holds more matrix work per barrier.  FP32 and FP64 use the single-problem
+// kernel's 64x64 tile and slab depth, so the k order (and the result) is the
+// same bit for bit.
 template <typename TI, typename TO>
 struct GroupedCfg;
 template <> struct GroupedCfg<float, float> { static constexpr int BM = 64, BN = 64, BK = 32, WM = 2, WN = 2; };
 template <> struct GroupedCfg<double, double> { static constexpr int BM = 64, BN = 64, BK = 16, WM = 2, WN = 2; };
-template <> struct GroupedCfg<uint16_t, float> { static constexpr int BM = 64, BN = 64, BK = 32, WM = 2, WN = 2; };
+template <> struct GroupedCfg<uint16_t, float> { static constexpr int BM = 64, BN = 64, BK = 64, WM = 2, WN = 2; };
 
 template <typename TI, typename TO>
 static int prepare(GroupedPlan &P) {
@@ -147,8 +152,14 @@ static int prepare(GroupedPlan &P) {
     return (int)e;
 }
 
-static int tile_bm_of(int dtype) { return 64; }
-static int tile_bn_of(int dtype) { return 64; }
+static int tile_bm_of(int dtype) {
+    return dtype == GROUPED_F64 ? GroupedCfg<double, double>::BM
+                                : (dtype == GROUPED_F32 ? GroupedCfg<float, float>::BM : GroupedCfg<uint16_t, float>::BM);
+}
+static int tile_bn_of(int dtype) {
+    return dtype == GROUPED_F64 ? GroupedCfg<double, double>::BN
+                                : (dtype == GROUPED_F32 ? GroupedCfg<float, float>::BN : GroupedCfg<uint16_t, float>::BN);
+}
 
 void *grouped_plan_create(int dtype, int count, const char *ta, const char *tb, const int *m, const int *n,
                           const int *k, const double *alpha, const void *const *A, const int *lda,
@@ -204,7 +215,7 @@ void *grouped_plan_create(int dtype, int count, const char *ta, const char *tb, 
         else if (dtype == GROUPED_F64) st = prepare<double, double>(*P);
         else st = prepare<uint16_t, float>(*P);
     }
-    // persistent grid: enough workgroups for every CU to hold several at once
+    // persistent grid: four resident workgroups per CU
     int64_t g = (int64_t)cus * 4;
     if (g > P->total_tiles) g = P->total_tiles;
     P->grid = (int)(g > 0 ? g : 1);

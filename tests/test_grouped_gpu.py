@@ -5,7 +5,8 @@ zgemm3m on the GPU.
   bound (FP32, FP64; bf16 operands are exact in FP32 products, so their bound
   is the FP32 one);
 * FP32/FP64 problems are bitwise equal to the single-problem kernel run with
-  the same 64x64 tile and no split-K (same k-slab order, same epilogue);
+  the same tile shape and slab depth and no split-K (same k order, same
+  epilogue);
 * a plan captured into a hipGraph and replayed several times, with eager work
   and synchronisations between the replays, reproduces the eager launch bit
   for bit every time;
@@ -25,7 +26,9 @@ from tests.util import opmat
 
 pytestmark = pytest.mark.gpu
 
-TILE_64x64 = 1
+# the grouped kernel's tile per dtype and the single-problem tile with the same shape and slab depth
+GROUP_TILE = {GROUPED_F32: (64, 64), GROUPED_BF16: (64, 64), GROUPED_F64: (64, 64)}
+SAME_SINGLE_TILE = {GROUPED_F32: 1, GROUPED_F64: 1}   # TILE_64x64
 
 
 def f32_to_bf16_bits(x):
@@ -99,7 +102,8 @@ def oracle_check(dtype, p, out):
 def test_grouped_vs_oracle(dtype):
     P = upload(random_problems(dtype, 40, seed=11 + dtype, lo=1, hi=300, zero_dims=True))
     g = GroupedGemm(dtype, P)
-    assert g.tiles == sum(-(-p["m"] // 64) * -(-p["n"] // 64) for p in P)
+    bm, bn = GROUP_TILE[dtype]
+    assert g.tiles == sum(-(-p["m"] // bm) * -(-p["n"] // bn) for p in P)
     g.run()
     synchronize()
     for p in P:
@@ -116,7 +120,7 @@ def test_grouped_bitwise_equals_single_gemm(dtype):
     for p in P:
         c2 = DeviceMatrix.from_padded(p["hc"], p["m"])
         single(p["transa"], p["transb"], p["m"], p["n"], p["k"], p["alpha"], p["a"], p["b"], p["beta"], c2,
-               tile=TILE_64x64, splitk=1)
+               tile=SAME_SINGLE_TILE[dtype], splitk=1)
         p["c_single"] = c2
     synchronize()
     for p in P:
@@ -192,7 +196,7 @@ def test_c4_512_problems_fp32_match_single_gemm_and_graph():
     bad = []
     for i, p in enumerate(P):
         c2 = DeviceMatrix(p["m"], p["n"], np.float32)
-        blas.sgemm(p["transa"], p["transb"], p["m"], p["n"], p["k"], 1.0, p["a"], p["b"], 0.0, c2, tile=TILE_64x64,
+        blas.sgemm(p["transa"], p["transb"], p["m"], p["n"], p["k"], 1.0, p["a"], p["b"], 0.0, c2, tile=SAME_SINGLE_TILE[GROUPED_F32],
                    splitk=1)
         if i % 16 == 0 or i < 4:
             if not np.array_equal(p["c"].to_host(), c2.to_host()):
