@@ -103,6 +103,39 @@ def gpu_leg(size, steps, warmup, variants=(("crt", 16), ("slices", 8)), seed=0):
     return out
 
 
+def accuracy_leg(size, ranges=(0, 8, 32), seed=7):
+    """Error vs exact (long-double) dot products at sampled entries of C for
+    A, B entries u * 2^e, e uniform in [-r, r]: native DGEMM, the CRT form at
+    14 / 16 moduli and the slice form at 8 slices (one call each)."""
+    from gemm_avx512f_amd import blas
+    from gemm_avx512f_amd.device import FILL_F64_RANGE, DeviceMatrix, Stream, fill_uniform
+
+    m = n = k = size
+    stream = Stream()
+    A, B, C = DeviceMatrix(m, k, np.float64), DeviceMatrix(k, n, np.float64), DeviceMatrix(m, n, np.float64)
+    rng = np.random.default_rng(999)
+    rows, cols = rng.integers(0, m, 256), rng.integers(0, n, 256)
+    out = {"workload": "DGEMM %d^3 NN, entries u*2^e, e uniform in [-r, r]; 256 sampled entries" % size}
+    runs = (("native_fp64", lambda: blas.dgemm("N", "N", m, n, k, 1.0, A, B, 0.0, C, stream=stream)),
+            ("crt_16_moduli", lambda: blas.dgemm_ozaki_crt("N", "N", m, n, k, 1.0, A, B, 0.0, C, nmoduli=16,
+                                                           stream=stream)),
+            ("crt_14_moduli", lambda: blas.dgemm_ozaki_crt("N", "N", m, n, k, 1.0, A, B, 0.0, C, nmoduli=14,
+                                                           stream=stream)),
+            ("slices_8", lambda: blas.dgemm_ozaki("N", "N", m, n, k, 1.0, A, B, 0.0, C, nslices=8, stream=stream)))
+    for r in ranges:
+        d = fill_uniform(A, seed + r, 0, kind=FILL_F64_RANGE, range_r=r, stream=stream)
+        fill_uniform(B, seed + r, d, kind=FILL_F64_RANGE, range_r=r, stream=stream)
+        a, b = A.to_host(), B.to_host()
+        res = {}
+        for name, fn in runs:
+            fn()
+            stream.synchronize()
+            e = _sampled_error(a, b, C.to_host(), rows, cols)
+            res[name] = {"max_rel": e[0], "rms_rel": e[1]}
+        out["r=%d" % r] = res
+    return out
+
+
 def sgemm_leg(size, steps, warmup, seed=0):
     """SGEMM size^3 NN: native FP32 matrix cores vs the bf16x3 / bf16x6 split
     forms on the same FP32 inputs, each with its sampled error."""
@@ -325,6 +358,8 @@ def main():
         ok = [v for v in variants if v["fp64_level"]] or variants
         result["emulated"] = max(ok, key=lambda v: v["tflops_fp64_equiv"])
         result["emulated_variants"] = variants
+    if not args.no_emulated:
+        result["accuracy_vs_range"] = accuracy_leg(size)
     if not args.no_sgemm:
         result["sgemm"] = sgemm_leg(size, args.steps, args.warmup)
     if not args.no_grouped:
