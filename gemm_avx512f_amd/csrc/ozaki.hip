@@ -512,13 +512,23 @@ typedef __attribute__((address_space(1))) void global_void_t;
 
 // DIRECT staging: global_load_lds_dwordx4 moves each 16-byte chunk from HBM/L2
 // straight into LDS (lane l of an instruction writes LDS base + 16 l), so no
-// VGPRs hold the next stage and no LDS store instructions stall the waves.
-// Rows are then unpadded (KB = 128 bytes) and chunk c of row r sits at position
-// c ^ ((r >> 1) & 7): the 16-byte fragment reads of 16 consecutive rows still
-// fall on 16 distinct 16-byte bank groups of the 256-byte bank width.
-template <int WM, int WN, int TM, int TN, int KB, bool DIRECT>
+// VGPRs hold the next stages and no LDS store instructions stall the waves;
+// NS stage buffers keep NS - 1 stages in flight.  Rows are unpadded (KB bytes)
+// and chunk c of row r sits at position c ^ sw(r), sw(r) = (r >> 1) & 7 for
+// 128-byte rows, (r >> 2) & 3 for 64-byte rows: the 16-byte fragment reads of
+// 16 consecutive rows fall on 16 distinct 16-byte groups of the 256-byte bank
+// width.
+__device__ __forceinline__ void wait_vmem_at_most(int n) {   // n wave-uniform
+    if (n >= 16) asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+    else if (n >= 8) asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+    else if (n >= 4) asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+    else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+template <int WM, int WN, int TM, int TN, int KB, bool DIRECT, int NS = 2>
 __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
-    static_assert(!DIRECT || KB == 128, "direct staging assumes 8 chunks of 16 bytes per row");
+    static_assert(!DIRECT || KB == 128 || KB == 64, "direct staging takes 64- or 128-byte rows");
+    static_assert(DIRECT || NS == 2, "register staging is double-buffered");
     constexpr int KP = DIRECT ? KB : KB + 16;
     constexpr int NT = WM * WN * 64;
     constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
@@ -570,18 +580,21 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
         }
     };
 
-    // direct staging: instruction j of wave w fills rows R0 = 8 (w + 8 j) ..
-    // R0 + 7 of the stage (A rows first, then B rows); lane l loads chunk
-    // (l & 7) ^ ((row >> 1) & 7) of row R0 + l / 8
-    constexpr int DROWS = (BM + BN) / (NT / 64);   // stage rows per wave
+    // direct staging: instruction j of wave w fills rows R0 = RPI (w + 8 j) ..
+    // R0 + RPI - 1 of the stage (A rows first, then B rows); lane l loads
+    // chunk (l % CPR) ^ sw(row) of row R0 + l / CPR
+    constexpr int CPR = KB / 16;                              // chunks per row
+    constexpr int RPI = 64 / CPR;                             // rows per load instruction
+    constexpr int LPW = (BM + BN) / RPI / (NT / 64);          // load instructions per wave per stage
+    auto swz = [](int row) { return CPR == 8 ? (row >> 1) & 7 : (row >> 2) & 3; };
     auto dload = [&](int st) {
         const int l = st / nkt, kt = st - l * nkt;
-        unsigned char *buf = smem + (st & 1) * STAGE;
+        unsigned char *buf = smem + (st % NS) * STAGE;
 #pragma unroll
-        for (int j = 0; j < DROWS / 8; ++j) {
-            const int r0 = 8 * (wave + (NT / 64) * j);
-            const int row = r0 + (lane >> 3);
-            const int c = (lane & 7) ^ ((row >> 1) & 7);
+        for (int j = 0; j < LPW; ++j) {
+            const int r0 = RPI * (wave + (NT / 64) * j);
+            const int row = r0 + lane / CPR;
+            const int c = (lane % CPR) ^ swz(row);
             const int8_t *src = row < BM
                 ? p.RA + (int64_t)l * p.planeA + (int64_t)(m0 + row) * p.Kp + kt * KB + 16 * c
                 : p.RB + (int64_t)l * p.planeB + (int64_t)(n0 + row - BM) * p.Kp + kt * KB + 16 * c;
@@ -592,13 +605,14 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
     const int q = lane & 31, g = lane >> 5;
     const int aoff = (wm0 + q) * KP + (DIRECT ? 0 : 16 * g);
     const int boff = (BM + wn0 + q) * KP + (DIRECT ? 0 : 16 * g);
-    const int qx = (q >> 1) & 7;   // swizzle of the lane's rows (32-row blocks keep it)
+    const int qx = swz(q);   // swizzle of the lane's rows (32-row blocks keep it)
     i16_t acc[TM][TN];
 
     if (total > 0) {
         if constexpr (DIRECT) {
-            dload(0);
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            const int pre = min(NS - 1, total);
+            for (int i = 0; i < pre; ++i) dload(i);
+            wait_vmem_at_most(LPW * (pre - 1));   // stage 0 has landed
         } else {
             gload(0);
             sstore(smem);
@@ -620,11 +634,13 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
 #pragma unroll
                     for (int e = 0; e < 16; ++e) acc[a][b][e] = 0;
             for (int i = c0; i < c1; ++i, ++st) {
-                const unsigned char *cur = smem + (st & 1) * STAGE;
+                const unsigned char *cur = smem + (DIRECT ? st % NS : st & 1) * STAGE;
                 unsigned char *nxt = smem + ((st + 1) & 1) * STAGE;
-                if (st + 1 < total) {
-                    if constexpr (DIRECT) dload(st + 1);
-                    else gload(st + 1);
+                if constexpr (DIRECT) {
+                    // the buffer of stage st + NS - 1 was last read in stage st - 1
+                    if (st + NS - 1 < total) dload(st + NS - 1);
+                } else {
+                    if (st + 1 < total) gload(st + 1);
                 }
                 if constexpr (DIRECT) {
                     // fragments of k-step ks + 1 are read while the matrix
@@ -670,8 +686,9 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
                     __builtin_amdgcn_sched_barrier(0);
                 }
                 if constexpr (DIRECT) {
-                    // this wave's chunks of the next stage have landed in LDS
-                    if (st + 1 < total) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                    // this wave's chunks of stage st + 1 have landed in LDS once
+                    // at most the loads of the stages after it are outstanding
+                    if (st + 1 < total) wait_vmem_at_most(LPW * (min(total - 1, st + NS - 1) - (st + 1)));
                 } else {
                     if (st + 1 < total) sstore(nxt);
                 }
@@ -704,6 +721,9 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
                 word |= (unsigned)(crt_mod(src[(w4 * 4 + e) * 64 + lane], m, im) & 0xFF) << (8 * e);
             Rv[((int64_t)l * (NACC / 4) + w4) * 64 + lane] = word;
         }
+        // the flush's stores count in the same vector-memory counter as the
+        // stage loads in flight: drain it so the counted waits stay exact
+        if constexpr (DIRECT) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
 
     // CRT reconstruction (a memory loop over the wave's residues)
@@ -865,14 +885,14 @@ static int crt_split(const double *X, int64_t rs, int64_t ks, int R, int K, int 
 
 // Tile layouts of the CRT kernel share one workspace layout: the int32 staging
 // and the residues are per output element (tile area), whatever the wave split.
-template <int WM, int WN, int TM, int TN, int KB, bool DIRECT = false>
+template <int WM, int WN, int TM, int TN, int KB, bool DIRECT = false, int NS = 2>
 static int launch_crt_variant(CrtParams &p, hipStream_t s) {
     constexpr int BM = WM * TM * 32, BN = WN * TN * 32;
     static_assert(BM == 256 && BN == 256, "rows are padded for 256-row tiles");
     p.tiles_m = (p.M + BM - 1) / BM;
     p.tiles_n = (p.N + BN - 1) / BN;
-    constexpr size_t lds = 2ull * (BM + BN) * (DIRECT ? KB : KB + 16);
-    auto kern = ozaki_crt_kernel<WM, WN, TM, TN, KB, DIRECT>;
+    constexpr size_t lds = (size_t)NS * (BM + BN) * (DIRECT ? KB : KB + 16);
+    auto kern = ozaki_crt_kernel<WM, WN, TM, TN, KB, DIRECT, NS>;
     static bool attr = false;
     if (!attr) {
         hipError_t he =
@@ -924,6 +944,7 @@ int launch_dgemm_ozaki_crt(bool ta, bool tb, int m, int n, int k, double alpha, 
     // FP64-equivalent for register staging at 8192^3, 16 moduli)
     if (v == 1) return launch_crt_variant<2, 4, 4, 2, 128>(p, s);   // register staging
     if (v == 2) return launch_crt_variant<2, 4, 4, 2, 64>(p, s);    // register staging, 64-deep stages
+    if (v == 4) return launch_crt_variant<2, 4, 4, 2, 64, true, 4>(p, s);   // direct, 4 stages of 64 bytes
     return launch_crt_variant<2, 4, 4, 2, 128, true>(p, s);
 }
 
