@@ -22,6 +22,8 @@ template <> struct Vec16<uint16_t> { using t = us8_t; static constexpr int N = 8
 //  v_mfma_f64_4x4x4_4b_f64: 4 independent blocks b of D_b(4x4) += A_b(4x4) B_b(4x4);
 //     A_b[i][k] at lane 4b+i+16k, B_b[k][j] at lane 4b+j+16k, D_b[i][j] at lane 16i+4b+j.
 //     (the 16x16x4 f64 form issues at ~60% of the FP64 peak on gfx950, this one at ~92%)
+//  v_mfma_f64_16x16x4_f64: A[i][k] at lane i+16k, B[k][j] at lane j+16k,
+//     D[i][j] at lane j+16*(i%4), element i/4.
 //  v_mfma_f32_32x32x2_f32: A[i][k] lane i+32k, B[k][j] lane j+32k,
 //     D[i][j] lane j+32*((i/4)%2), element (i/8)*4 + i%4.
 //  v_mfma_f32_32x32x16_bf16 / v_mfma_i32_32x32x32_i8: lane l supplies 8 (bf16) or

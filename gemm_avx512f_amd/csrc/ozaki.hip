@@ -36,7 +36,6 @@ static constexpr int OZ_ROWPAD = 768;            // rows padded to a multiple of
 // LDS rows are KB + 16 bytes (KB = k bytes per stage, 64 or 128): the 16-byte
 // fragment reads of 16 consecutive rows land on distinct 16-byte bank groups
 // (80*q and 144*q mod 256 are permutations of 16*q).
-static constexpr int OZ_MAXPAIRS = 36;            // s = 8
 static constexpr int OZ_FLUSH_K = 524160;         // 524160 * 64^2 < 2^31 - 1
 
 int ozaki_pad_rows(int x) { return (x + OZ_ROWPAD - 1) / OZ_ROWPAD * OZ_ROWPAD; }

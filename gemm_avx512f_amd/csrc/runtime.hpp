@@ -39,7 +39,7 @@ DevBuf &buf(BufId id);                     // for host-pointer calls (under rt()
 // hold several at once (e.g. DTRSM's inverse blocks while its trailing GEMMs
 // use the split-K slot).  Growth synchronises the device first, so memory a
 // queued kernel may still use is never freed under it.
-enum WsSlot { WS_GEMM = 0, WS_TRSM_INV, WS_TRSM_TMP, WS_EMU_A, WS_EMU_B, WS_EMU_AUX, WS_EMU_W, WS_COMPLEX, WS_GROUPED,
+enum WsSlot { WS_GEMM = 0, WS_TRSM_INV, WS_EMU_A, WS_EMU_B, WS_EMU_AUX, WS_EMU_W, WS_COMPLEX, WS_GROUPED,
               WS_TUNE_A, WS_TUNE_B, WS_TUNE_C, WS_MLP,
               WS_COUNT };
 void *device_workspace(int slot, size_t bytes, int *status);

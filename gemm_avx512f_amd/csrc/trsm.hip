@@ -6,81 +6,106 @@
 // X overwrites B.  Only the uplo triangle of A is read; diag 'U' never reads
 // the diagonal.  Plan (replaces dtrsm_16x4's register solves,
 // OpenBLAS-like_implementation/dtrsm_16x4/dtrsm_kernel_16x4_skylakex.S):
-//   1. one kernel inverts every NB x NB diagonal block of op(A) in parallel
+//   1. one kernel inverts every NB x NB diagonal block in parallel
 //      (column-parallel substitution against the identity; the reference
 //      likewise works from inverted pivots, :1480-1484);
 //   2. B <- alpha B;
 //   3. recursive halving over the triangle: solve the leading half, update the
-//      trailing half with one large GEMM (the FP64 MFMA kernel), solve the
-//      trailing half; a leaf (<= NB) is X_b = inv(T_bb) * B_b, a GEMM into a
-//      scratch panel copied back.
+//      trailing half with one large GEMM (the FP64 matrix-core kernel), solve
+//      the trailing half.  A leaf of up to NL rows is one launch of the panel
+//      kernel: each workgroup holds a slab of WC right-hand sides in LDS and
+//      walks the leaf's NB-row sub-blocks, X_q = inv(T_qq) B_q followed by the
+//      update of the leaf's remaining rows, in place.
+// Side R is solved in the same "left" form: X op(A) = B  <=>  op(A)^T X^T = B^T,
+// so the panel kernel and the diagonal inverses see T = op(A)^T and B^T
+// through strides; the trailing updates stay GEMMs on B as stored.
 #include "common.hpp"
 #include "gemm_kernels.hpp"
 #include "trsm.hpp"
+#include "tuner.hpp"
 
 namespace gemm_amd {
 
-static constexpr int TRSM_NB = 64;
+static constexpr int TRSM_NB = 64;    // inverted diagonal block
+static constexpr int TRSM_NL = 256;   // rows solved by one panel launch
 
 // Element (r, c) of op(A) from column-major storage.
 __device__ __forceinline__ double opa(const double *A, int64_t lda, bool trans, int64_t r, int64_t c) {
     return trans ? A[c + r * lda] : A[r + c * lda];
 }
 
-// grid.x = diagonal block index; 256 threads = 64 columns of the identity x 4
-// row groups of 16 rows.  W[b] (NB x NB, ld NB) <- inv(op(A)_bb); rows/cols past
-// the matrix edge are the identity.
-__global__ __launch_bounds__(256) void trsm_diag_inverse_kernel(const double *__restrict__ A, int64_t lda, int na,
-                                                                bool trans, bool oplower, bool unit,
-                                                                double *__restrict__ W) {
+// grid.x = diagonal block index; one wave per block, lane c computes column c
+// of the inverse by substitution against e_c with the column in registers and
+// the block's rows read from LDS (every lane reads the same entry: a
+// broadcast).  W[b] (NB x NB, ld NB) <- inv(op(A)_bb); rows/cols past the
+// matrix edge are the identity.  Block 0 also writes the NB zeros that follow
+// the last block.
+__global__ __launch_bounds__(64) void trsm_diag_inverse_kernel(const double *__restrict__ A, int64_t lda, int na,
+                                                               bool trans, bool oplower, bool unit,
+                                                               double *__restrict__ W) {
     constexpr int NB = TRSM_NB;
-    __shared__ double T[NB][NB + 1];   // op(A) block, [row][col]
-    __shared__ double X[NB][NB + 1];   // solution, [row][col]
+    // op(A) block [row][col], later the inverse; the pitch keeps rows 16-byte
+    // aligned for paired broadcast reads and spreads column accesses over banks
+    __shared__ __align__(16) double T[NB][NB + 2];
     __shared__ double piv[NB];
     const int b = blockIdx.x;
     const int64_t base = (int64_t)b * NB;
     const int nb = min(NB, na - (int)base);
-    const int tid = threadIdx.x, c = tid & 63, rg = tid >> 6;
-    for (int idx = tid; idx < NB * NB; idx += 256) {
-        const int r = idx % NB, cc = idx / NB;
-        double v = 0.0;
-        const bool in_tri = oplower ? (r > cc) : (r < cc);
-        if (r < nb && cc < nb && in_tri) v = opa(A, lda, trans, base + r, base + cc);
-        T[r][cc] = v;
-        X[r][cc] = (r == cc) ? 1.0 : 0.0;
+    const int c = threadIdx.x;
+    for (int c0 = 0; c0 < NB; c0 += 16) {   // lane = the contiguous index; 16 loads in flight
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int r = trans ? c0 + u : c, col = trans ? c : c0 + u;
+            const bool in_tri = oplower ? (r > col) : (r < col);
+            v[u] = (r < nb && col < nb && in_tri) ? opa(A, lda, trans, base + r, base + col) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int r = trans ? c0 + u : c, col = trans ? c : c0 + u;
+            T[r][col] = v[u];
+        }
     }
-    if (tid < NB) {
+    {
         double d = 1.0;
-        if (tid < nb && !unit) d = opa(A, lda, trans, base + tid, base + tid);
-        piv[tid] = 1.0 / d;
+        if (c < nb && !unit) d = opa(A, lda, trans, base + c, base + c);
+        piv[c] = 1.0 / d;
     }
     __syncthreads();
+    double x[NB];
     if (oplower) {
+#pragma unroll
         for (int i = 0; i < NB; ++i) {
-            if (rg == (i >> 4)) X[i][c] *= piv[i];
-            __syncthreads();
-            const double xi = X[i][c];
-#pragma unroll 4
-            for (int r = rg * 16; r < rg * 16 + 16; ++r)
-                if (r > i) X[r][c] -= T[r][i] * xi;
-            __syncthreads();
+            double s0 = (i == c) ? 1.0 : 0.0, s1 = 0.0;
+#pragma unroll
+            for (int j = 0; j + 1 < i; j += 2) {
+                s0 = fma(-T[i][j], x[j], s0);
+                s1 = fma(-T[i][j + 1], x[j + 1], s1);
+            }
+            if (i & 1) s0 = fma(-T[i][i - 1], x[i - 1], s0);
+            x[i] = (s0 + s1) * piv[i];
         }
     } else {
+#pragma unroll
         for (int i = NB - 1; i >= 0; --i) {
-            if (rg == (i >> 4)) X[i][c] *= piv[i];
-            __syncthreads();
-            const double xi = X[i][c];
-#pragma unroll 4
-            for (int r = rg * 16; r < rg * 16 + 16; ++r)
-                if (r < i) X[r][c] -= T[r][i] * xi;
-            __syncthreads();
+            double s0 = (i == c) ? 1.0 : 0.0, s1 = 0.0;
+            const int n = NB - 1 - i;   // terms j = i+1 .. NB-1
+#pragma unroll
+            for (int t = 0; t + 1 < n; t += 2) {
+                s0 = fma(-T[i][i + 1 + t], x[i + 1 + t], s0);
+                s1 = fma(-T[i][i + 2 + t], x[i + 2 + t], s1);
+            }
+            if (n & 1) s0 = fma(-T[i][NB - 1], x[NB - 1], s0);
+            x[i] = (s0 + s1) * piv[i];
         }
     }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NB; ++r) T[r][c] = x[r];
+    __syncthreads();
     double *out = W + (int64_t)b * NB * NB;
-    for (int idx = tid; idx < NB * NB; idx += 256) {
-        const int r = idx % NB, cc = idx / NB;
-        out[r + (int64_t)cc * NB] = X[r][cc];
-    }
+    for (int cc = 0; cc < NB; ++cc) out[c + (int64_t)cc * NB] = T[c][cc];
+    if (b == 0) W[(int64_t)gridDim.x * NB * NB + c] = 0.0;   // the zero operand of the panel kernel
 }
 
 // B (rows x cols, ldb) <- alpha * B; alpha == 0 writes zeros without reading.
@@ -93,13 +118,151 @@ __global__ __launch_bounds__(256) void trsm_scale_kernel(int rows, int cols, dou
     }
 }
 
-// dst (rows x cols, ldd) <- src (rows x cols, lds)
-__global__ __launch_bounds__(256) void copy_block_kernel(int rows, int cols, const double *__restrict__ src,
-                                                         int64_t lds, double *__restrict__ dst, int64_t ldd) {
-    const int64_t total = (int64_t)rows * cols;
-    for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
-        const int64_t i = idx % rows, j = idx / rows;
-        dst[i + j * ldd] = src[i + j * lds];
+// Solve T Y = C on rows [p0, p0+len) of C (len <= NL, p0 a multiple of NB),
+// in place, where T(r, c) = A[r*ars + c*acs] is lower (forward) or upper
+// (backward) and C(i, j) = B[i*brs + j*bcs] has ncols columns.  Rows outside
+// the leaf are final and already folded into C.  W holds inv(T_bb) for every
+// NB block (column-major NB x NB, identity past the edge), followed by NB
+// zeros (`zero`).  grid.x = column slabs of 4*NCG; one of ars/acs and one of
+// brs/bcs is 1 (the coalesced side).
+//
+// 16 waves; wave w keeps rows [16w, 16w+16) of the slab, NCG groups of 4
+// columns, in v_mfma_f64_4x4x4_4b accumulators: the instruction's 4 blocks
+// are 4 row quads of the tile sharing one B operand, so that A[r][k] sits at
+// lane r+16k (r = row in the tile), B[k][j] at lane 4b+j+16k for every b, and
+// D[r][j] at lane 16(r%4)+4(r/4)+j.  Sub-block q (waves 4q..4q+3): its waves
+// publish C_q to LDS, compute X_q = inv(T_qq) C_q (the inverse rows straight
+// from registers) and publish -X_q; the waves of the rows still to solve then
+// accumulate C_w += T_wq (-X_q) (T_wq straight from registers).  Every wave's
+// operands for the next sub-block are fetched while the current one computes.
+// Columns past ncols are zero and stay zero.
+template <int NCG>
+__global__ __launch_bounds__(1024) void trsm_panel_kernel(const double *__restrict__ A, int64_t ars, int64_t acs,
+                                                          const double *__restrict__ W,
+                                                          const double *__restrict__ zero, double *__restrict__ B,
+                                                          int64_t brs, int64_t bcs, int p0, int len, int ncols,
+                                                          bool lower) {
+    constexpr int NT = 1024, NB = TRSM_NB, NL = TRSM_NL, WC = 4 * NCG, SP = WC + 1;
+    static_assert(NB == 64 && NL == 256, "one 16-row tile per wave, 4 waves per NB block");
+    __shared__ double S[NL * SP];    // slab staging for coalesced global access
+    __shared__ double Cq[NB * SP];   // C_q, published by the sub-block's waves
+    __shared__ double Xq[NB * SP];   // -X_q, read by the waves still to solve
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4;
+    const int ar = lane & 15;                            // A operand: tile row
+    const int bj = lane & 3;                             // B operand: column in the group
+    const int dr = 4 * ((lane >> 2) & 3) + (lane >> 4);  // D: tile row (column bj)
+    const int j0 = blockIdx.x * WC;
+    const int nq = (len + NB - 1) / NB;
+    const bool brow = (brs == 1), arow = (ars == 1);
+
+    // 1: this wave's rows are sub-block q; 2: they are still to solve; 0: idle
+    auto role = [&](int q) {
+        const int blk = w >> 2;
+        if (16 * w >= len) return 0;
+        if (blk == q) return 1;
+        return (lower ? blk > q : blk < q) ? 2 : 0;
+    };
+    // k of the s-th instruction for the lanes of group g: four consecutive k
+    // per instruction where T's rows are contiguous, sixteen consecutive k per
+    // lane where its columns are
+    auto kof = [&](bool rows_contig, int s) { return rows_contig ? 4 * s + g : 16 * g + s; };
+    // One branch-free stream of 16 loads per wave and sub-block, so that the
+    // loads for the next sub-block stay in flight behind the current one's
+    // work.  Entries past the matrix edge (and idle waves) read a zero.
+    auto fetch = [&](int q, double *dst) {
+        const int rl = role(q);
+        const int row = 16 * w + ar, kc = NB * q;
+        const double *p = zero;
+        int64_t st = 0;
+        int kmax = NB, kmul = 0, kadd = 0;   // entry s valid iff s * kmul + kadd < kmax
+        if (rl == 1) {   // inverse rows 16(w&3) + ar of block q (column-major NB x NB)
+            p = W + ((int64_t)(p0 / NB) + q) * (NB * NB) + 16 * (w & 3) + ar + g * NB;
+            st = 4 * NB;
+        } else if (rl == 2) {   // T(row, NB q + k)
+            p = A + (int64_t)(p0 + row) * ars + (int64_t)(p0 + kc) * acs;
+            if (arow) {
+                p += g * acs;
+                st = 4 * acs;
+                kmul = 4;
+                kadd = g;
+            } else {
+                p += 16 * g;
+                st = 1;
+                kmul = 1;
+                kadd = 16 * g;
+            }
+            kmax = row < len ? len - kc : 0;
+        }
+#pragma unroll
+        for (int s = 0; s < 16; ++s) dst[s] = *(s * kmul + kadd < kmax ? p + s * st : zero);
+    };
+
+    double pc[16], pn[16];
+    int q = lower ? 0 : nq - 1;
+    fetch(q, pn);   // in flight together with the slab
+    {
+        constexpr int SL = (NL * WC + NT - 1) / NT;
+        double sv[SL];
+#pragma unroll
+        for (int i = 0; i < SL; ++i) {
+            const int e = i * NT + tid;
+            const int r = brow ? e % NL : e / WC, c = brow ? e / NL : e % WC;
+            sv[i] = *(r < len && j0 + c < ncols ? B + (int64_t)(p0 + r) * brs + (int64_t)(j0 + c) * bcs : zero);
+        }
+#pragma unroll
+        for (int i = 0; i < SL; ++i) {
+            const int e = i * NT + tid;
+            const int r = brow ? e % NL : e / WC, c = brow ? e / NL : e % WC;
+            S[r * SP + c] = sv[i];
+        }
+    }
+    __syncthreads();
+    double acc[NCG];
+#pragma unroll
+    for (int cg = 0; cg < NCG; ++cg) acc[cg] = S[(16 * w + dr) * SP + 4 * cg + bj];
+
+    for (int step = 0; step < nq; ++step) {
+        const int rl = role(q);
+#pragma unroll
+        for (int s = 0; s < 16; ++s) pc[s] = pn[s];   // this sub-block's operands (landed by now)
+        const int qn = lower ? q + 1 : q - 1;
+        if (step + 1 < nq) fetch(qn, pn);
+        if (rl == 1) {
+#pragma unroll
+            for (int cg = 0; cg < NCG; ++cg) Cq[(16 * (w & 3) + dr) * SP + 4 * cg + bj] = acc[cg];
+        }
+        __syncthreads();
+        if (rl == 1) {
+#pragma unroll
+            for (int cg = 0; cg < NCG; ++cg) {
+                double x = 0.0;
+#pragma unroll
+                for (int s = 0; s < 16; ++s) x = mfma_f64_4x4x4(pc[s], Cq[kof(true, s) * SP + 4 * cg + bj], x);
+                acc[cg] = x;
+                Xq[(16 * (w & 3) + dr) * SP + 4 * cg + bj] = -x;   // published negated
+            }
+        }
+        __syncthreads();
+        if (rl == 2) {
+#pragma unroll
+            for (int cg = 0; cg < NCG; ++cg) {
+#pragma unroll
+                for (int s = 0; s < 16; ++s)
+                    acc[cg] = mfma_f64_4x4x4(pc[s], Xq[kof(arow, s) * SP + 4 * cg + bj], acc[cg]);
+            }
+        }
+        q = qn;
+    }
+
+#pragma unroll
+    for (int cg = 0; cg < NCG; ++cg) S[(16 * w + dr) * SP + 4 * cg + bj] = acc[cg];
+    __syncthreads();
+    constexpr int SL = (NL * WC + NT - 1) / NT;
+#pragma unroll
+    for (int i = 0; i < SL; ++i) {
+        const int e = i * NT + tid;
+        const int r = brow ? e % NL : e / WC, c = brow ? e / NL : e % WC;
+        if (e < NL * WC && r < len && j0 + c < ncols) B[(int64_t)(p0 + r) * brs + (int64_t)(j0 + c) * bcs] = S[r * SP + c];
     }
 }
 
@@ -117,8 +280,7 @@ struct TrsmCtx {
     int64_t lda;
     double *B;
     int64_t ldb;
-    const double *W;     // inverted diagonal blocks
-    double *T;           // leaf scratch (NB x n or m x NB)
+    const double *W;     // inverted diagonal blocks of the left-form triangle
     void *gemm_work;     // split-K workspace for the trailing GEMMs
     size_t gemm_work_bytes;
     int cus;
@@ -128,8 +290,10 @@ struct TrsmCtx {
     // op(A)[r0.., c0..] as a GEMM operand: pointer and transpose flag
     const double *opa_ptr(int64_t r0, int64_t c0) const { return trans ? A + c0 + r0 * lda : A + r0 + c0 * lda; }
 
-    GemmConfig cfg(int mm, int nn, int kk) const {
-        GemmConfig c = pick_trsm_gemm_config(mm, nn, kk, cus);
+    // The trailing updates take the configuration every DGEMM call gets (the
+    // learned picker, or its fallback), limited to the split-K workspace.
+    GemmConfig cfg(bool ta, int mm, int nn, int kk, int64_t la) const {
+        GemmConfig c = pick_gemm_config(DT_F64, ta, false, mm, nn, kk, (int)(la - (ta ? kk : mm)), cus);
         if (gemm_workspace_bytes(mm, nn, kk, c, 8) > gemm_work_bytes) c.splitk = 1;
         return c;
     }
@@ -137,38 +301,38 @@ struct TrsmCtx {
     void gemm(bool ta, bool tb, int mm, int nn, int kk, double alpha, const double *a, int64_t la, const double *b,
               int64_t lb, double beta, double *c, int64_t lc) {
         if (err || mm <= 0 || nn <= 0) return;
-        err = launch_dgemm(ta, tb, mm, nn, kk, alpha, a, la, b, lb, beta, c, lc, cfg(mm, nn, kk), gemm_work, s);
+        err = launch_dgemm(ta, tb, mm, nn, kk, alpha, a, la, b, lb, beta, c, lc, cfg(ta, mm, nn, kk, la), gemm_work,
+                           s);
     }
 
+    // Leaf: rows [p0, p0+len) of the left-form system T Y = C.  Side L: T =
+    // op(A), C = B.  Side R: T = op(A)^T, C = B^T (columns of B).
     void leaf(int p0, int len) {
         if (err) return;
-        const double *inv = W + (int64_t)(p0 / TRSM_NB) * TRSM_NB * TRSM_NB;
-        if (left) {   // B[p0:p0+len, :] <- inv * B[p0:p0+len, :]
-            gemm(false, false, len, n, len, 1.0, inv, TRSM_NB, B + p0, ldb, 0.0, T, len);
-            if (!err) {
-                hipLaunchKernelGGL(copy_block_kernel, dim3(grid_for((int64_t)len * n)), dim3(256), 0, s, len, n, T,
-                                   (int64_t)len, B + p0, ldb);
-                err = (int)hipGetLastError();
-            }
-        } else {      // B[:, p0:p0+len] <- B[:, p0:p0+len] * inv
-            gemm(false, false, m, len, len, 1.0, B + (int64_t)p0 * ldb, ldb, inv, TRSM_NB, 0.0, T, m);
-            if (!err) {
-                hipLaunchKernelGGL(copy_block_kernel, dim3(grid_for((int64_t)m * len)), dim3(256), 0, s, m, len, T,
-                                   (int64_t)m, B + (int64_t)p0 * ldb, ldb);
-                err = (int)hipGetLastError();
-            }
-        }
+        const bool tt = left ? trans : !trans;   // T(r, c) = tt ? A[c + r lda] : A[r + c lda]
+        const int64_t ars = tt ? lda : 1, acs = tt ? 1 : lda;
+        const int64_t brs = left ? 1 : ldb, bcs = left ? ldb : 1;
+        const int ncols = left ? n : m;
+        const bool lower = left ? oplower : !oplower;
+        // 4-column slabs while they fit the grid in one round of workgroups
+        const int ncg = (ncols + 3) / 4 <= cus ? 1 : 2;
+        const int grid = (ncols + 4 * ncg - 1) / (4 * ncg);
+        const double *zero = W + (int64_t)((left ? m : n) + TRSM_NB - 1) / TRSM_NB * (TRSM_NB * TRSM_NB);
+        auto kern = ncg == 1 ? trsm_panel_kernel<1> : trsm_panel_kernel<2>;
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(1024), 0, s, A, ars, acs, W, zero, B, brs, bcs, p0, len, ncols,
+                           lower);
+        err = (int)hipGetLastError();
     }
 
     // Solve the diagonal range [p0, p0+len) of op(A).
     void rec(int p0, int len) {
         if (err || len <= 0) return;
-        if (len <= TRSM_NB) {
+        if (len <= TRSM_NL) {
             leaf(p0, len);
             return;
         }
-        int h = ((len / 2 + TRSM_NB - 1) / TRSM_NB) * TRSM_NB;   // split on a block boundary
-        if (h >= len) h = len - TRSM_NB;
+        int h = ((len / 2 + TRSM_NL - 1) / TRSM_NL) * TRSM_NL;   // split on a leaf boundary
+        if (h >= len) h = len - TRSM_NL;
         const int p1 = p0 + h, len2 = len - h;
         if (left) {
             if (oplower) {           // forward: X1, then B2 -= L21 X1, then X2
@@ -200,37 +364,12 @@ struct TrsmCtx {
 
 size_t trsm_workspace_inv_bytes(int na) {
     const int64_t nblk = (na + TRSM_NB - 1) / TRSM_NB;
-    return (size_t)nblk * TRSM_NB * TRSM_NB * sizeof(double);
-}
-
-size_t trsm_workspace_tmp_bytes(bool left, int m, int n) {
-    return (size_t)TRSM_NB * (size_t)(left ? n : m) * sizeof(double);
-}
-
-GemmConfig pick_trsm_gemm_config(int m, int n, int k, int cus) {
-    // the same occupancy rule as plain GEMM, but never a 256-row tile: the
-    // trailing updates are short-and-wide panels
-    GemmConfig c;
-    const long t128 = (long)((m + 127) / 128) * ((n + 127) / 128);
-    const long t64 = (long)((m + 63) / 64) * ((n + 63) / 64);
-    if (t128 >= cus) {
-        c.tile = TILE_128x128;
-    } else {
-        c.tile = TILE_64x64;
-        if (t64 < cus && k >= 512) {
-            int want = (int)((2L * cus + t64 - 1) / t64);
-            int maxk = k / 256;
-            if (want > maxk) want = maxk;
-            if (want > 8) want = 8;
-            c.splitk = want > 1 ? want : 1;
-        }
-    }
-    return c;
+    return ((size_t)nblk * TRSM_NB * TRSM_NB + TRSM_NB) * sizeof(double);
 }
 
 int launch_dtrsm(bool left, bool upper, bool trans, bool unit, int m, int n, double alpha, const double *A,
-                 int64_t lda, double *B, int64_t ldb, double *inv_ws, double *tmp_ws, void *gemm_ws,
-                 size_t gemm_ws_bytes, int cus, hipStream_t s) {
+                 int64_t lda, double *B, int64_t ldb, double *inv_ws, void *gemm_ws, size_t gemm_ws_bytes, int cus,
+                 hipStream_t s) {
     if (m <= 0 || n <= 0) return 0;
     if (alpha == 0.0) {
         hipLaunchKernelGGL(trsm_scale_kernel, dim3(grid_for((int64_t)m * n)), dim3(256), 0, s, m, n, 0.0, B, ldb);
@@ -238,8 +377,11 @@ int launch_dtrsm(bool left, bool upper, bool trans, bool unit, int m, int n, dou
     }
     const int na = left ? m : n;
     const bool oplower = (!upper) != trans;
+    // Inverses of the left-form triangle's diagonal blocks: op(A) for side L,
+    // op(A)^T (the other transpose, the other triangle) for side R.
+    const bool inv_trans = left ? trans : !trans, inv_lower = left ? oplower : !oplower;
     const int nblk = (na + TRSM_NB - 1) / TRSM_NB;
-    hipLaunchKernelGGL(trsm_diag_inverse_kernel, dim3(nblk), dim3(256), 0, s, A, lda, na, trans, oplower, unit,
+    hipLaunchKernelGGL(trsm_diag_inverse_kernel, dim3(nblk), dim3(64), 0, s, A, lda, na, inv_trans, inv_lower, unit,
                        inv_ws);
     if (hipError_t e = hipGetLastError()) return (int)e;
     if (alpha != 1.0) {
@@ -257,7 +399,6 @@ int launch_dtrsm(bool left, bool upper, bool trans, bool unit, int m, int n, dou
     ctx.B = B;
     ctx.ldb = ldb;
     ctx.W = inv_ws;
-    ctx.T = tmp_ws;
     ctx.gemm_work = gemm_ws;
     ctx.gemm_work_bytes = gemm_ws ? gemm_ws_bytes : 0;
     ctx.cus = cus;

@@ -139,9 +139,9 @@ Picker &picker() {
 
 int ensure_capacity(Picker &P, int batch) {
     if (batch <= P.cap) return 0;
-    if (P.d_x) hipFree(P.d_x);
-    if (P.d_y) hipFree(P.d_y);
-    if (P.d_ws) hipFree(P.d_ws);
+    if (P.d_x) (void)hipFree(P.d_x);
+    if (P.d_y) (void)hipFree(P.d_y);
+    if (P.d_ws) (void)hipFree(P.d_ws);
     P.d_x = P.d_y = P.d_ws = nullptr;
     P.cap = 0;
     int cap = batch < 64 ? 64 : batch;
@@ -184,13 +184,13 @@ int infer_locked(Picker &P, int count, const float *feat, float *out) {
     // the inference runs on the picker's own stream; relaxed capture mode keeps
     // it legal while a caller's stream is being captured into a graph
     hipStreamCaptureMode mode = hipStreamCaptureModeRelaxed;
-    hipThreadExchangeStreamCaptureMode(&mode);
+    (void)hipThreadExchangeStreamCaptureMode(&mode);
     hipError_t e = hipMemcpyAsync(P.d_x, feat, sizeof(float) * count * MLP_IN, hipMemcpyHostToDevice, P.stream);
     int r = e == hipSuccess ? mlp_forward(P.d_params, P.d_x, count, P.d_y, P.d_ws, P.stream) : (int)e;
     if (!r) e = hipMemcpyAsync(out, P.d_y, sizeof(float) * count * MLP_OUT, hipMemcpyDeviceToHost, P.stream);
     if (!r && e == hipSuccess) e = hipStreamSynchronize(P.stream);
     if (!r) r = (int)e;
-    hipThreadExchangeStreamCaptureMode(&mode);
+    (void)hipThreadExchangeStreamCaptureMode(&mode);
     return r;
 }
 

@@ -304,10 +304,15 @@ def _main(argv):
         ms = sweep(a.op, bool(a.ta), bool(a.tb), a.m, a.n, a.k, a.pad)
         fl = 2.0 * a.m * a.n * a.k * (3 if a.op == OP_CGEMM3M else 1)
         cands = candidates()
-        cells = ["%s tile %d split %d: %.1f" % ("*" if c == int(np.argmin(np.where(ms[:NCFG] > 0, ms[:NCFG], np.inf)))
-                                                else " ", cands[c][0], cands[c][1], fl / (ms[c] * 1e-3) / 1e12)
+        best = int(np.argmin(np.where(ms[:NCFG] > 0, ms[:NCFG], np.inf)))
+        one = np.zeros(1, dtype=SHAPE_DTYPE)
+        one["op"], one["ta"], one["tb"], one["m"], one["n"], one["k"], one["pad"] = (
+            a.op, a.ta, a.tb, a.m, a.n, a.k, a.pad)
+        learned = int(picker_choose(one)[0])
+        cells = ["%s%s tile %d split %d: %.1f" % ("*" if c == best else " ", "P" if c == learned else " ",
+                                                  cands[c][0], cands[c][1], fl / (ms[c] * 1e-3) / 1e12)
                  for c in range(NCFG) if ms[c] > 0]
-        log("%s ta=%d tb=%d %dx%dx%d pad %d (TFLOP/s; * = fastest; heuristic's choice %.1f):" % (
+        log("%s ta=%d tb=%d %dx%dx%d pad %d (TFLOP/s; * = fastest, P = learned picker; heuristic's choice %.1f):" % (
             OP_NAMES[a.op], a.ta, a.tb, a.m, a.n, a.k, a.pad, fl / (ms[NCFG] * 1e-3) / 1e12))
         for c in cells:
             log("  " + c)

@@ -48,9 +48,13 @@ def residual(side, uplo, transa, diag, a, x, b, alpha, m, n):
     return r / scale
 
 
+@pytest.mark.parametrize("shape", [(300, 170), (777, 45), (45, 777), (9, 3)])
 @pytest.mark.parametrize("side,uplo,transa,diag", VARIANTS)
-def test_dtrsm_variants_vs_oracle(side, uplo, transa, diag):
-    m, n = 300, 170          # several 64-blocks, ragged edges, recursion depth > 1
+def test_dtrsm_variants_vs_oracle(side, uplo, transa, diag, shape):
+    # 300 x 170: a full 256-row leaf plus a ragged one; 777: three leaves and a
+    # partial 64-block, with few (side L) or many (side R) slabs of right-hand
+    # sides; 9 x 3: a single partial block
+    m, n = shape
     a, b = tri_system(side, uplo, m, n, seed=VARIANTS.index((side, uplo, transa, diag)) + 300, boost=float(max(m, n)), pad=3)
     x = solve_gpu(side, uplo, transa, diag, m, n, 1.25, a, b)
     assert np.isfinite(x[:m, :n]).all()
