@@ -164,6 +164,63 @@ def sgemm_leg(size, steps, warmup, seed=0):
     return out
 
 
+def configs_leg(steps=20, warmup=3, seed=0):
+    """The smaller workloads next to the headline: SGEMM 1024^3 NN (alpha 1,
+    beta 0); DGEMM 4096^3 NN and TN with alpha 1.5, beta 0.5; DTRSM side L,
+    lower, no transpose, non-unit, M=4096, N=512, triangle uniform[-1,1),
+    diagonal 4096 + uniform[0,1).  Each with its sampled error (GEMMs) or
+    residual ||A X - B|| / (||A|| ||X|| M) (DTRSM)."""
+    from gemm_avx512f_amd import blas
+    from gemm_avx512f_amd.device import DeviceMatrix, Stream, fill_uniform
+
+    stream = Stream()
+    rng = np.random.default_rng(2468)
+    out = {}
+    n = 1024
+    A, B, C = (DeviceMatrix(n, n, np.float32) for _ in range(3))
+    d = fill_uniform(A, seed, 0, stream=stream)
+    fill_uniform(B, seed, d, stream=stream)
+    ms, _ = _time(lambda: blas.sgemm("N", "N", n, n, n, 1.0, A, B, 0.0, C, stream=stream), stream, steps, warmup)
+    rows, cols = rng.integers(0, n, 256), rng.integers(0, n, 256)
+    err = _sampled_error(A.to_host().astype(np.float64), B.to_host().astype(np.float64),
+                         C.to_host().astype(np.float64), rows, cols)
+    out["sgemm_1024_nn"] = {"tflops": round(2.0 * n ** 3 * steps / (ms / 1e3) / 1e12, 2),
+                            "ms_per_call": round(ms / steps, 4), "max_rel": err[0], "rms_rel": err[1]}
+    n = 4096
+    A, B, C = (DeviceMatrix(n, n, np.float64) for _ in range(3))
+    d = fill_uniform(A, seed, 0, stream=stream)
+    d = fill_uniform(B, seed, d, stream=stream)
+    fill_uniform(C, seed, d, stream=stream)
+    a, b, c0 = A.to_host(), B.to_host(), C.to_host()
+    rows, cols = rng.integers(0, n, 256), rng.integers(0, n, 256)
+    for ta in ("N", "T"):
+        C.buf.upload(np.asfortranarray(c0))
+        blas.dgemm(ta, "N", n, n, n, 1.5, A, B, 0.5, C, stream=stream)   # one checked call from the known C
+        stream.synchronize()
+        opa = a if ta == "N" else a.T
+        ref = np.array([1.5 * np.dot(opa[i, :].astype(np.longdouble), b[:, j].astype(np.longdouble))
+                        + 0.5 * np.longdouble(c0[i, j]) for i, j in zip(rows, cols)])
+        got = C.to_host()[rows, cols].astype(np.longdouble)
+        ms, _ = _time(lambda: blas.dgemm(ta, "N", n, n, n, 1.5, A, B, 0.5, C, stream=stream), stream, steps, warmup)
+        out["dgemm_4096_%sn" % ta.lower()] = {
+            "tflops": round(2.0 * n ** 3 * steps / (ms / 1e3) / 1e12, 2), "ms_per_call": round(ms / steps, 4),
+            "max_rel": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))}
+    m, nr = 4096, 512
+    tri = np.tril(np.random.default_rng(seed + 11).uniform(-1.0, 1.0, (m, m)))
+    tri[np.arange(m), np.arange(m)] = 4096.0 + np.random.default_rng(seed + 12).uniform(0.0, 1.0, m)
+    rhs = np.random.default_rng(seed + 13).uniform(-1.0, 1.0, (m, nr))
+    T, X = DeviceMatrix.from_host(np.asfortranarray(tri)), DeviceMatrix.from_host(np.asfortranarray(rhs))
+    blas.dtrsm("L", "L", "N", "N", m, nr, 1.0, T, X, stream=stream)
+    stream.synchronize()
+    x = X.to_host()
+    resid = float(np.max(np.abs(tri @ x - rhs)) / (np.max(np.abs(tri)) * np.max(np.abs(x)) * m))
+    ms, _ = _time(lambda: blas.dtrsm("L", "L", "N", "N", m, nr, 1.0, T, X, stream=stream), stream, steps, warmup)
+    out["dtrsm_llnn_4096x512"] = {"tflops": round(float(m) * m * nr * steps / (ms / 1e3) / 1e12, 2),
+                                  "ms_per_call": round(ms / steps, 4), "residual": resid,
+                                  "flops": "M^2 N"}
+    return out
+
+
 def c4_shapes(seed, count=512):
     """M, N, K log-uniform over [32, 2048] rounded to multiples of 8; transposes drawn per problem."""
     rng = np.random.default_rng(seed)
@@ -286,6 +343,8 @@ def main():
     ap.add_argument("--cpu-size", type=int, default=None, help="CPU leg problem size (default: --size)")
     ap.add_argument("--no-grouped", action="store_true", help="skip the grouped-GEMM / zgemm3m batch leg")
     ap.add_argument("--no-sgemm", action="store_true", help="skip the SGEMM / bf16x split leg")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the SGEMM 1024^3 / DGEMM 4096^3 / DTRSM 4096x512 leg")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB first (a known-size kernel that "
                          "calibrates the HBM byte counters when the run is profiled)")
@@ -364,6 +423,8 @@ def main():
         result["sgemm"] = sgemm_leg(size, args.steps, args.warmup)
     if not args.no_grouped:
         result["batching"] = grouped_leg()
+    if not args.no_configs:
+        result["configs"] = configs_leg()
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_leg(args.cpu_size or size)
