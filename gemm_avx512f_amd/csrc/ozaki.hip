@@ -537,7 +537,9 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
     constexpr int NACC = TM * TN * 16;            // int32 accumulators per lane
     static_assert(BM * CH % NT == 0 && BN * CH % NT == 0, "stage must split evenly over the workgroup");
     extern __shared__ __align__(16) unsigned char smem[];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the wave index is uniform: keeping it (and what derives from it) in
+    // scalar registers leaves the vector registers to the accumulators
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm0 = (wave % WM) * (TM * 32), wn0 = (wave / WM) * (TN * 32);
     int tm, tn;
     tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, 8, tm, tn);
@@ -584,20 +586,27 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
     // chunk (l % CPR) ^ sw(row) of row R0 + l / CPR
     constexpr int CPR = KB / 16;                              // chunks per row
     constexpr int RPI = 64 / CPR;                             // rows per load instruction
-    constexpr int LPW = (BM + BN) / RPI / (NT / 64);          // load instructions per wave per stage
+    constexpr int NW = NT / 64;                               // waves
+    constexpr int LPW = (BM + BN) / RPI / NW;                 // load instructions per wave per stage
+    constexpr int JA = BM / (RPI * NW);                       // of which fill A rows
+    static_assert(BM % (RPI * NW) == 0 && NW % 2 == 0, "whole A/B instructions, even wave count");
     auto swz = [](int row) { return CPR == 8 ? (row >> 1) & 7 : (row >> 2) & 3; };
+    // lane l of instruction j loads row RPI (wave + NW j) + l / CPR; with an
+    // even wave count its swizzle, and so its byte offset in the row, does not
+    // depend on j: one per-lane offset plus a uniform base per instruction
+    const int lrow = lane / CPR;
+    const int loff = lrow * p.Kp + 16 * ((lane % CPR) ^ swz(RPI * wave + lrow));
     auto dload = [&](int st) {
         const int l = st / nkt, kt = st - l * nkt;
         unsigned char *buf = smem + (st % NS) * STAGE;
+        const int8_t *ua = p.RA + (int64_t)l * p.planeA + (int64_t)(m0 + RPI * wave) * p.Kp + kt * KB;
+        const int8_t *ub = p.RB + (int64_t)l * p.planeB + (int64_t)(n0 + RPI * wave) * p.Kp + kt * KB;
+        const int64_t js = (int64_t)(RPI * NW) * p.Kp;
 #pragma unroll
         for (int j = 0; j < LPW; ++j) {
-            const int r0 = RPI * (wave + (NT / 64) * j);
-            const int row = r0 + lane / CPR;
-            const int c = (lane % CPR) ^ swz(row);
-            const int8_t *src = row < BM
-                ? p.RA + (int64_t)l * p.planeA + (int64_t)(m0 + row) * p.Kp + kt * KB + 16 * c
-                : p.RB + (int64_t)l * p.planeB + (int64_t)(n0 + row - BM) * p.Kp + kt * KB + 16 * c;
-            __builtin_amdgcn_global_load_lds((global_void_t *)src, (lds_void_t *)(buf + r0 * KP), 16, 0, 0);
+            const int8_t *u = j < JA ? ua + j * js : ub + (j - JA) * js;
+            __builtin_amdgcn_global_load_lds((global_void_t *)(u + loff), (lds_void_t *)(buf + RPI * (wave + NW * j) * KP),
+                                             16, 0, 0);
         }
     };
 
@@ -641,9 +650,9 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
                 } else {
                     if (st + 1 < total) gload(st + 1);
                 }
-                if constexpr (DIRECT) {
+                if constexpr (DIRECT && TM * TN <= 8) {
                     // fragments of k-step ks + 1 are read while the matrix
-                    // instructions of k-step ks run
+                    // instructions of k-step ks run (register room permitting)
                     i4_t av[2][TM], bv[2][TN];
                     auto frag = [&](int ks, i4_t *a, i4_t *b) {
                         const int ko = ((2 * ks + g) ^ qx) << 4;
@@ -693,8 +702,27 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
                 }
                 __syncthreads();
             }
-            // the accumulators leave through plain int32 stores (an opaque
-            // base keeps the store addresses from being hoisted into VGPRs)
+            if (!multi) {
+                // one exact chunk: residues straight from the accumulators,
+                // four per word, in accumulator order
+                unsigned *rv = Rv + (int64_t)l * (NACC / 4) * 64 + lane;
+#pragma unroll
+                for (int a = 0; a < TM; ++a)
+#pragma unroll
+                    for (int b = 0; b < TN; ++b)
+#pragma unroll
+                        for (int e4 = 0; e4 < 4; ++e4) {
+                            unsigned word = 0;
+#pragma unroll
+                            for (int e = 0; e < 4; ++e)
+                                word |= (unsigned)(crt_mod(acc[a][b][4 * e4 + e], m, im) & 0xFF) << (8 * e);
+                            rv[((a * TN + b) * 4 + e4) * 64] = word;
+                        }
+                continue;
+            }
+            // several chunks: the accumulators leave through plain int32
+            // stores (an opaque base keeps the store addresses from being
+            // hoisted into VGPRs) and fold into the running residue
             int *wb = Wi + lane;
             __asm__ volatile("" : "+v"(wb));
 #pragma unroll
@@ -703,22 +731,19 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
                 for (int b = 0; b < TN; ++b)
 #pragma unroll
                     for (int e = 0; e < 16; ++e) wb[((a * TN + b) * 16 + e) * 64] = acc[a][b][e];
-            if (multi) {
-                // fold this chunk into the running residue (a memory loop)
-                for (int idx = 0; idx < NACC; ++idx) {
-                    const int prev = c0 == 0 ? 0 : Wc[idx * 64 + lane];
-                    Wc[idx * 64 + lane] = crt_mod(prev + Wi[idx * 64 + lane], m, im);
-                }
+            for (int idx = 0; idx < NACC; ++idx) {
+                const int prev = c0 == 0 ? 0 : Wc[idx * 64 + lane];
+                Wc[idx * 64 + lane] = crt_mod(prev + Wi[idx * 64 + lane], m, im);
             }
         }
-        const int *src = multi ? Wc : Wi;
-        // residues of this modulus, four per word, in accumulator order
-        for (int w4 = 0; w4 < NACC / 4; ++w4) {
-            unsigned word = 0;
+        if (multi) {
+            // residues of this modulus, four per word, in accumulator order
+            for (int w4 = 0; w4 < NACC / 4; ++w4) {
+                unsigned word = 0;
 #pragma unroll
-            for (int e = 0; e < 4; ++e)
-                word |= (unsigned)(crt_mod(src[(w4 * 4 + e) * 64 + lane], m, im) & 0xFF) << (8 * e);
-            Rv[((int64_t)l * (NACC / 4) + w4) * 64 + lane] = word;
+                for (int e = 0; e < 4; ++e) word |= (unsigned)(Wc[(w4 * 4 + e) * 64 + lane] & 0xFF) << (8 * e);
+                Rv[((int64_t)l * (NACC / 4) + w4) * 64 + lane] = word;
+            }
         }
         // the flush's stores count in the same vector-memory counter as the
         // stage loads in flight: drain it so the counted waits stay exact
@@ -944,6 +969,7 @@ int launch_dgemm_ozaki_crt(bool ta, bool tb, int m, int n, int k, double alpha, 
     if (v == 1) return launch_crt_variant<2, 4, 4, 2, 128>(p, s);   // register staging
     if (v == 2) return launch_crt_variant<2, 4, 4, 2, 64>(p, s);    // register staging, 64-deep stages
     if (v == 4) return launch_crt_variant<2, 4, 4, 2, 64, true, 4>(p, s);   // direct, 4 stages of 64 bytes
+    if (v == 5) return launch_crt_variant<2, 2, 4, 4, 128, true>(p, s);     // direct, 4 waves of 128x128
     return launch_crt_variant<2, 4, 4, 2, 128, true>(p, s);
 }
 
