@@ -443,6 +443,7 @@ struct CrtConst {
     double w[OZ_CRT_MAXMOD][4];   // limbs of W_l, low first, each < 2^32
     double Mk[4];                 // limbs of M
     double invM;                  // 1 / M (rounded)
+    double c32[OZ_CRT_MAXMOD];    // 2^32 mod m_l
 };
 
 static constexpr int OZ_CRT_CHUNK_K = 131008;   // 131008 * 128^2 < 2^31
@@ -462,21 +463,28 @@ __global__ __launch_bounds__(256) void oz_residue_kernel(const double *__restric
     int e = 0;
     if (amax > 0.0) frexp(amax, &e);
     if (blockIdx.x == 0 && (tid & 3) == 0) exps[r0 + rr] = e - al;
-    double x[16];
+    // x = rint(a 2^(al-e)) (|x| <= 2^al, an exact integer) is split once as
+    // x = xh 2^32 + xl with 0 <= xl < 2^32 (both exact); per modulus
+    // y = xh (2^32 mod m) + xl is exact, |y| < 2^33, so rint(y/m) is the
+    // exact nearest quotient (ties only at m = 256, where either residue
+    // gives the same byte) and y - m rint(y/m) is the symmetric residue
+    double xh[16], xl[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) x[i] = rint(ldexp(T[rr][kq + i], al - e));   // |x| <= 2^al, exact integer
+    for (int i = 0; i < 16; ++i) {
+        const double x = rint(ldexp(T[rr][kq + i], al - e));
+        xh[i] = floor(x * 0x1p-32);
+        xl[i] = fma(-xh[i], 0x1p32, x);
+    }
     const int64_t plane = (int64_t)Rp * Kp;
     int8_t *dst = out + (int64_t)(r0 + rr) * Kp + k0 + kq;
     const int nmod = cc->nmod;
     for (int l = 0; l < nmod; ++l) {
-        const double m = (double)cc->m[l], im = cc->invm[l], h = 0.5 * m;
+        const double m = (double)cc->m[l], im = cc->invm[l], c = cc->c32[l];
         i4_t w = {0, 0, 0, 0};
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            // x - m*rint(x/m) is exact (fma); rint(x/m) can be off by one for
-            // |x| near 2^56, which one correction absorbs
-            double r = fma(-rint(x[i] * im), m, x[i]);
-            r = r > h ? r - m : (r < -h ? r + m : r);
+            const double y = fma(xh[i], c, xl[i]);
+            const double r = fma(-rint(y * im), m, y);
             w[i >> 2] |= ((int)r & 0xFF) << (8 * (i & 3));
         }
         *reinterpret_cast<i4_t *>(dst + l * plane) = w;
@@ -868,6 +876,7 @@ static const CrtConst *crt_device_constants(int nmod, int *status) {
     }
     for (int k = 0; k < 4; ++k) c.Mk[k] = h.Mk[k];
     c.invM = h.invM;
+    for (int l = 0; l < nmod; ++l) c.c32[l] = (double)((1ull << 32) % (unsigned long long)h.m[l]);
     void *d = nullptr;
     hipError_t e = hipMalloc(&d, sizeof(CrtConst));   // kept for the life of the process
     if (e == hipSuccess) e = hipMemcpy(d, &c, sizeof(CrtConst), hipMemcpyHostToDevice);
