@@ -675,12 +675,16 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
 #pragma unroll
                     for (int ks = 0; ks < KB / 32; ++ks) {
                         if (ks + 1 < KB / 32) frag(ks + 1, av[(ks + 1) & 1], bv[(ks + 1) & 1]);
+                        // the scheduler would otherwise sink each read next to
+                        // its first use and stall the matrix pipe on LDS latency
+                        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                         for (int a = 0; a < TM; ++a)
 #pragma unroll
                             for (int b = 0; b < TN; ++b)
                                 acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[ks & 1][a], bv[ks & 1][b],
                                                                                   acc[a][b], 0, 0, 0);
+                        __builtin_amdgcn_sched_barrier(0);
                     }
                 } else
 #pragma unroll
