@@ -156,18 +156,27 @@ __global__ __launch_bounds__(WM *WN * 64) void bf16x_gemm_kernel(BxParams p) {
         const unsigned char *cur = smem + (st & 1) * STAGE;
         unsigned char *nxt = smem + ((st + 1) & 1) * STAGE;
         if (st + 1 < total) gload(st + 1);
+        // 16 k values (32 bytes) per MFMA step; the fragments of step ks + 1
+        // are read while the matrix instructions of step ks run (the sched
+        // barriers keep the scheduler from sinking each read next to its use,
+        // which stalls the matrix pipe on LDS latency)
+        bf8_t av[2][TM], bv[2][TN];
+        auto frag = [&](int ks, bf8_t *a, bf8_t *b) {
 #pragma unroll
-        for (int ks = 0; ks < KB / 32; ++ks) {   // 16 k values (32 bytes) per MFMA step
-            bf8_t av[TM], bv[TN];
+            for (int s = 0; s < TM; ++s) a[s] = *reinterpret_cast<const bf8_t *>(cur + aoff + 32 * s * KP + 32 * ks);
 #pragma unroll
-            for (int s = 0; s < TM; ++s) av[s] = *reinterpret_cast<const bf8_t *>(cur + aoff + 32 * s * KP + 32 * ks);
+            for (int s = 0; s < TN; ++s) b[s] = *reinterpret_cast<const bf8_t *>(cur + boff + 32 * s * KP + 32 * ks);
+        };
+        frag(0, av[0], bv[0]);
 #pragma unroll
-            for (int s = 0; s < TN; ++s) bv[s] = *reinterpret_cast<const bf8_t *>(cur + boff + 32 * s * KP + 32 * ks);
+        for (int ks = 0; ks < KB / 32; ++ks) {
+            if (ks + 1 < KB / 32) frag(ks + 1, av[(ks + 1) & 1], bv[(ks + 1) & 1]);
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int a = 0; a < TM; ++a)
 #pragma unroll
                 for (int b = 0; b < TN; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[ks & 1][a], bv[ks & 1][b], acc[a][b], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
         if (st + 1 < total) sstore(nxt);

@@ -234,20 +234,27 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_gemm_kernel(OzParams p) {
                 const unsigned char *cur = smem + (st & 1) * STAGE;
                 unsigned char *nxt = smem + ((st + 1) & 1) * STAGE;
                 if (st + 1 < total) gload(st + 1);
+                // the fragments of k-step ks + 1 are read while the matrix
+                // instructions of k-step ks run (the sched barriers keep the
+                // reads from being sunk next to their uses)
+                i4_t av[2][TM], bv[2][TN];
+                auto frag = [&](int ks, i4_t *a, i4_t *b) {
+#pragma unroll
+                    for (int s = 0; s < TM; ++s) a[s] = *reinterpret_cast<const i4_t *>(cur + aoff + 32 * s * KP + 32 * ks);
+#pragma unroll
+                    for (int s = 0; s < TN; ++s) b[s] = *reinterpret_cast<const i4_t *>(cur + boff + 32 * s * KP + 32 * ks);
+                };
+                frag(0, av[0], bv[0]);
 #pragma unroll
                 for (int ks = 0; ks < KB / 32; ++ks) {
-                    i4_t av[TM], bv[TN];
-#pragma unroll
-                    for (int s = 0; s < TM; ++s)
-                        av[s] = *reinterpret_cast<const i4_t *>(cur + aoff + 32 * s * KP + 32 * ks);
-#pragma unroll
-                    for (int s = 0; s < TN; ++s)
-                        bv[s] = *reinterpret_cast<const i4_t *>(cur + boff + 32 * s * KP + 32 * ks);
+                    if (ks + 1 < KB / 32) frag(ks + 1, av[(ks + 1) & 1], bv[(ks + 1) & 1]);
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int a = 0; a < TM; ++a)
 #pragma unroll
                         for (int b = 0; b < TN; ++b)
-                            acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[a], bv[b], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av[ks & 1][a], bv[ks & 1][b], acc[a][b],
+                                                                              0, 0, 0);
                     __builtin_amdgcn_sched_barrier(0);
                 }
                 if (st + 1 < total) sstore(nxt);
