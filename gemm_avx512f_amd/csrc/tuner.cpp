@@ -194,13 +194,21 @@ int infer_locked(Picker &P, int count, const float *feat, float *out) {
     return r;
 }
 
+// The valid candidate with the lowest predicted cost; a split-K choice gives
+// way to the same tile without split-K when the network predicts that one
+// within PICK_TIE (log2 units, ~1.4%): no reduce pass, no workspace, and the
+// network's resolution is a few percent.
+constexpr float PICK_TIE = 0.02f;
 int argmin_valid(int op, int m, int n, int k, const float *y) {
     int best = -1;
     for (int c = 0; c < TUNE_NCFG; ++c) {
         if (!tune_valid(op, m, n, k, tune_candidate(c))) continue;
         if (best < 0 || y[c] < y[best]) best = c;
     }
-    return best < 0 ? 0 : best;
+    if (best < 0) return 0;
+    const int one = (best / TUNE_NSPLIT) * TUNE_NSPLIT;   // c = tile * TUNE_NSPLIT + log2(split-K)
+    if (one != best && tune_valid(op, m, n, k, tune_candidate(one)) && y[one] - y[best] <= PICK_TIE) best = one;
+    return best;
 }
 
 }  // namespace

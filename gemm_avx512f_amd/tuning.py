@@ -49,6 +49,23 @@ def candidate_index(tile, splitk):
     return tile * 5 + int(np.log2(splitk))
 
 
+PICK_TIE = 0.02   # log2 units: see choose()
+
+
+def choose(y, valid):
+    """Candidate chosen from predicted costs y (count x NCFG), the rule the
+    library applies after inference: the valid minimum, except that a split-K
+    candidate gives way to the same tile without split-K when that one is
+    predicted within PICK_TIE (no reduce pass, no workspace; the network's
+    resolution is a few percent)."""
+    yy = np.where(valid, y, np.inf)
+    best = np.argmin(yy, axis=1)
+    rows = np.arange(len(best))
+    one = (best // 5) * 5
+    take = valid[rows, one] & (yy[rows, one] - yy[rows, best] <= PICK_TIE)
+    return np.where(take, one, best)
+
+
 def valid_mask(op, m, n, k):
     """Candidates the library considers valid (each K partition >= 64 deep, partials <= 1 GiB)."""
     m, n, k = (np.asarray(v, dtype=np.float64) for v in (m, n, k))

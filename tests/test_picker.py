@@ -168,7 +168,7 @@ def test_picker_regret_within_5pct_and_beats_heuristic():
     shapes, ms = d["shapes"], d["ms"]
     y = O.mlp_forward(np.load(WEIGHTS)["params"].astype(np.float64), tuning.features(shapes).astype(np.float64))
     valid = tuning.valid_mask(shapes["op"], shapes["m"], shapes["n"], shapes["k"])
-    choice = np.argmin(np.where(valid, y, np.inf), axis=1)
+    choice = tuning.choose(y, valid)
     r_mlp = tuning.regret(ms, choice)
     r_heur = tuning.regret(ms, np.full(len(ms), 20))
     assert len(ms) >= 2000
