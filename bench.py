@@ -106,7 +106,7 @@ def gpu_leg(size, steps, warmup, variants=(("crt", 16), ("slices", 8)), seed=0):
 def accuracy_leg(size, ranges=(0, 8, 32), seed=7):
     """Error vs exact (long-double) dot products at sampled entries of C for
     A, B entries u * 2^e, e uniform in [-r, r]: native DGEMM, the CRT form at
-    14 / 16 moduli and the slice form at 8 slices (one call each)."""
+    12..16 moduli and the slice form at 4..8 slices (one call each)."""
     from gemm_avx512f_amd import blas
     from gemm_avx512f_amd.device import FILL_F64_RANGE, DeviceMatrix, Stream, fill_uniform
 
@@ -116,12 +116,13 @@ def accuracy_leg(size, ranges=(0, 8, 32), seed=7):
     rng = np.random.default_rng(999)
     rows, cols = rng.integers(0, m, 256), rng.integers(0, n, 256)
     out = {"workload": "DGEMM %d^3 NN, entries u*2^e, e uniform in [-r, r]; 256 sampled entries" % size}
-    runs = (("native_fp64", lambda: blas.dgemm("N", "N", m, n, k, 1.0, A, B, 0.0, C, stream=stream)),
-            ("crt_16_moduli", lambda: blas.dgemm_ozaki_crt("N", "N", m, n, k, 1.0, A, B, 0.0, C, nmoduli=16,
-                                                           stream=stream)),
-            ("crt_14_moduli", lambda: blas.dgemm_ozaki_crt("N", "N", m, n, k, 1.0, A, B, 0.0, C, nmoduli=14,
-                                                           stream=stream)),
-            ("slices_8", lambda: blas.dgemm_ozaki("N", "N", m, n, k, 1.0, A, B, 0.0, C, nslices=8, stream=stream)))
+    runs = [("native_fp64", lambda: blas.dgemm("N", "N", m, n, k, 1.0, A, B, 0.0, C, stream=stream))]
+    for nm in (16, 15, 14, 13, 12):
+        runs.append(("crt_%d_moduli" % nm, lambda nm=nm: blas.dgemm_ozaki_crt("N", "N", m, n, k, 1.0, A, B, 0.0, C,
+                                                                            nmoduli=nm, stream=stream)))
+    for ns in (8, 7, 6, 5, 4):
+        runs.append(("slices_%d" % ns, lambda ns=ns: blas.dgemm_ozaki("N", "N", m, n, k, 1.0, A, B, 0.0, C,
+                                                                     nslices=ns, stream=stream)))
     for r in ranges:
         d = fill_uniform(A, seed + r, 0, kind=FILL_F64_RANGE, range_r=r, stream=stream)
         fill_uniform(B, seed + r, d, kind=FILL_F64_RANGE, range_r=r, stream=stream)
@@ -339,7 +340,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-emulated", action="store_true", help="skip the emulated (int8) DGEMM legs")
     ap.add_argument("--nslices", type=int, default=8, help="slices per operand of the slice-form emulated DGEMM")
-    ap.add_argument("--nmoduli", type=int, default=16, help="moduli of the modular-form emulated DGEMM")
+    ap.add_argument("--nmoduli", type=int, default=16,
+                    help="moduli of the modular-form emulated DGEMM (timed at this count and one fewer)")
     ap.add_argument("--cpu-size", type=int, default=None, help="CPU leg problem size (default: --size)")
     ap.add_argument("--no-grouped", action="store_true", help="skip the grouped-GEMM / zgemm3m batch leg")
     ap.add_argument("--no-sgemm", action="store_true", help="skip the SGEMM / bf16x split leg")
@@ -363,7 +365,7 @@ def main():
         _lib.check(_lib.lib().gemm_amd_memcpy(dst.ptr, src.ptr, nb, 3), "calibration copy")
         synchronize()
         del src, dst
-    variants = () if args.no_emulated else (("crt", args.nmoduli), ("slices", args.nslices))
+    variants = () if args.no_emulated else (("crt", args.nmoduli), ("crt", args.nmoduli - 1), ("slices", args.nslices))
     legs = gpu_leg(size, args.steps, args.warmup, variants=variants)
     ms, wall = legs["native"]["ms"], legs["native"]["wall"]
     flops = 2.0 * size ** 3
