@@ -32,11 +32,12 @@ __global__ __launch_bounds__(WM *WN * 64, (sizeof(T) == 8 && (BM / WM) * (BN / W
 void gemm_mfma_kernel(GemmParams<T> p) {
     constexpr int NT = WM * WN * 64;
     constexpr int WTM = BM / WM, WTN = BN / WN;
-    using LA = TileLoader<T, BM, BK, NT, TA, VEC>;    // op(A) rows = M; k-contiguous iff A is transposed
-    using LB = TileLoader<T, BN, BK, NT, !TB, VEC>;   // op(B) rows = N; k-contiguous iff B is not transposed
-    using Wave = typename WaveSel<T, WTM, WTN, BK>::type;
-    constexpr int PITCH = LA::PITCH;
-    constexpr int STAGE = (BM + BN) * PITCH;
+    using Lay = SlabLayout<T, TA, TB, BM, BN, BK>;
+    using LA = TileLoader<T, BM, BK, NT, TA, VEC, Lay::KMA>;   // op(A) rows = M; k-contiguous iff A is transposed
+    using LB = TileLoader<T, BN, BK, NT, !TB, VEC>;            // op(B) rows = N; k-contiguous iff B is not transposed
+    using Wave = typename Lay::template Wave<WTM, WTN>;
+    constexpr int STAGE = Lay::STAGE;
+    constexpr int BOFF = LA::SIZE;
 
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *smem = reinterpret_cast<T *>(smem_raw);
@@ -74,7 +75,7 @@ void gemm_mfma_kernel(GemmParams<T> p) {
             lb.load(bbase, p.ldb, nlim, kend - kbeg, tid);
         }
         la.store(smem, tid);
-        lb.store(smem + BM * PITCH, tid);
+        lb.store(smem + BOFF, tid);
     }
     __syncthreads();
     if (interior) {
@@ -86,10 +87,10 @@ void gemm_mfma_kernel(GemmParams<T> p) {
                 la.load_full(abase + (kt + 1) * astep, p.lda, tid);
                 lb.load_full(bbase + (kt + 1) * bstep, p.ldb, tid);
             }
-            w.compute(cur, cur + BM * PITCH);
+            w.compute(cur, cur + BOFF);
             if (more) {
                 la.store(nxt, tid);
-                lb.store(nxt + BM * PITCH, tid);
+                lb.store(nxt + BOFF, tid);
             }
             __syncthreads();
         }
@@ -103,10 +104,10 @@ void gemm_mfma_kernel(GemmParams<T> p) {
                 la.load(abase + (kt + 1) * astep, p.lda, mlim, krem, tid);
                 lb.load(bbase + (kt + 1) * bstep, p.ldb, nlim, krem, tid);
             }
-            w.compute(cur, cur + BM * PITCH);
+            w.compute(cur, cur + BOFF);
             if (more) {
                 la.store(nxt, tid);
-                lb.store(nxt + BM * PITCH, tid);
+                lb.store(nxt + BOFF, tid);
             }
             __syncthreads();
         }
@@ -186,8 +187,7 @@ size_t gemm_workspace_bytes(int m, int n, int, const GemmConfig &cfg, int elem_b
 
 template <typename T, bool TA, bool TB, int BM, int BN, int WM, int WN, int BK, bool VEC>
 static hipError_t launch_one(const GemmParams<T> &p, int splits, hipStream_t s) {
-    constexpr int PITCH = BK + 16 / (int)sizeof(T);
-    constexpr size_t lds = 2ull * (BM + BN) * PITCH * sizeof(T);
+    constexpr size_t lds = 2ull * SlabLayout<T, TA, TB, BM, BN, BK>::STAGE * sizeof(T);
     auto kern = gemm_mfma_kernel<T, TA, TB, BM, BN, BK, WM, WN, VEC>;
     static bool attr_set = false;   // per instantiation; set once per process
     if (!attr_set) {

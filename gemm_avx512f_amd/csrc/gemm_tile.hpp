@@ -2,6 +2,7 @@
 // the single-problem kernel (gemm_mfma.hip) and the grouped kernel (grouped.hip).
 #pragma once
 #include "common.hpp"
+#include <type_traits>
 
 namespace gemm_amd {
 
@@ -35,13 +36,22 @@ struct GemmParams {
 // the transposing store: its 4-way conflicts cost less than the doubled load
 // instruction count of a k-pair gather (measured 57.1 vs 53.7 TFLOP/s at
 // 8192^3 NN).
-template <typename T, int ROWS, int BK, int NT, bool KCONTIG, bool VEC>
+//
+// KMAJOR (FP64 rows-contiguous operands only): the slab is stored [k][row]
+// with a row pitch of KPITCH = ROWS + 4 doubles instead, so the 16-byte
+// register of two neighbouring rows lands as one conflict-free LDS store
+// (the transposing store of the [row][k] layout puts 8 lanes on each bank
+// pair).
+template <typename T, int ROWS, int BK, int NT, bool KCONTIG, bool VEC, bool KMAJOR = false>
 struct TileLoader {
     using V = typename Vec16<T>::t;
     static constexpr int VW = Vec16<T>::N;
     static constexpr int PITCH = BK + VW;
+    static constexpr int KPITCH = ROWS + 4;
+    static constexpr int SIZE = KMAJOR ? BK * KPITCH : ROWS * PITCH;   // elements per stage
     static constexpr int NVEC = ROWS * BK / VW;
     static constexpr bool KGATHER = !KCONTIG && sizeof(T) == 4;
+    static_assert(!KMAJOR || (!KCONTIG && sizeof(T) == 8), "k-major slabs are for rows-contiguous FP64 operands");
     static_assert(NVEC % NT == 0, "tile must split evenly over the workgroup");
     static constexpr int PER = NVEC / NT;
     V reg[PER];
@@ -110,7 +120,9 @@ struct TileLoader {
         for (int i = 0; i < PER; ++i) {
             int r, k;
             coords(tid + i * NT, r, k);
-            if (KCONTIG || KGATHER) {
+            if (KMAJOR) {
+                *reinterpret_cast<V *>(lds + k * KPITCH + r) = reg[i];
+            } else if (KCONTIG || KGATHER) {
                 *reinterpret_cast<V *>(lds + r * PITCH + k) = reg[i];
             } else {
 #pragma unroll
@@ -130,18 +142,27 @@ struct TileLoader {
 // measurably starves the FP64 matrix pipe on gfx950, LDS reads do not.  Lane
 // group g = lane/16 owns k = g*BK/4 .. (g+1)*BK/4-1 of each LDS slab, read two
 // at a time (any k assignment is valid as long as A and B use the same one).
-template <int WTM, int WTN, int BK>
+//
+// APK > 0: the A slab is k-major with row pitch APK, and A sub-tiles come in
+// row pairs: lane q reads rows 2q, 2q+1 of a 32-row band at one k (one
+// 16-byte read) and feeds the first to sub-tile 2p (the band's even rows) and
+// the second to sub-tile 2p+1 (its odd rows), both against the same B
+// fragment; the two k of a k pair are two such reads.
+template <int WTM, int WTN, int BK, int APK = 0>
 struct WaveF64 {
     static constexpr int SM = WTM / 16, SN = WTN / 16;
     static constexpr int PITCH = BK + 2;
     static_assert(BK % 8 == 0, "FP64 slab depth must be a multiple of 8");
+    static_assert(APK == 0 || SM % 2 == 0, "row pairs need an even number of A sub-tiles");
     double acc[SM][SN][4];
     int aoff[SM], boff[SN][4];
 
     __device__ __forceinline__ void init(int lane, int wm0, int wn0) {
         const int q = lane & 15, g = lane >> 4, b = q >> 2, j = q & 3;
 #pragma unroll
-        for (int sm = 0; sm < SM; ++sm) aoff[sm] = (wm0 + sm * 16 + q) * PITCH + (BK / 4) * g;
+        for (int sm = 0; sm < SM; ++sm)
+            aoff[sm] = APK ? (BK / 4) * g * APK + wm0 + 32 * (sm >> 1) + 2 * q
+                           : (wm0 + sm * 16 + q) * PITCH + (BK / 4) * g;
 #pragma unroll
         for (int sn = 0; sn < SN; ++sn)
 #pragma unroll
@@ -159,8 +180,17 @@ struct WaveF64 {
 #pragma unroll
         for (int h = 0; h < BK / 8; ++h) {
             d2_t av[SM], bv[SN][4];
+            if (APK) {
+                // av[2p + e] = rows (2q, 2q+1) of band p at k = 2h + e
 #pragma unroll
-            for (int sm = 0; sm < SM; ++sm) av[sm] = *reinterpret_cast<const d2_t *>(As + aoff[sm] + 2 * h);
+                for (int p = 0; p < SM / 2; ++p)
+#pragma unroll
+                    for (int e = 0; e < 2; ++e)
+                        av[2 * p + e] = *reinterpret_cast<const d2_t *>(As + aoff[2 * p] + (2 * h + e) * APK);
+            } else {
+#pragma unroll
+                for (int sm = 0; sm < SM; ++sm) av[sm] = *reinterpret_cast<const d2_t *>(As + aoff[sm] + 2 * h);
+            }
 #pragma unroll
             for (int sn = 0; sn < SN; ++sn)
 #pragma unroll
@@ -172,8 +202,11 @@ struct WaveF64 {
 #pragma unroll
                     for (int sn = 0; sn < SN; ++sn)
 #pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            acc[sm][sn][r] = mfma_f64_4x4x4(av[sm][e], bv[sn][r][e], acc[sm][sn][r]);
+                        for (int r = 0; r < 4; ++r) {
+                            // k-major: sub-tile sm = 2p + hf takes element hf of the band's read at k = 2h + e
+                            const double a = APK ? av[2 * (sm >> 1) + e][sm & 1] : av[sm][e];
+                            acc[sm][sn][r] = mfma_f64_4x4x4(a, bv[sn][r][e], acc[sm][sn][r]);
+                        }
             // keep each phase's fragment reads next to its matrix ops: hoisting
             // every phase's reads to the top would need 4x the fragment registers
             __builtin_amdgcn_sched_barrier(0);
@@ -190,7 +223,8 @@ struct WaveF64 {
             for (int sn = 0; sn < SN; ++sn)
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    visit(wm0 + sm * 16 + 4 * b + i, wn0 + sn * 16 + 4 * ((b + r) & 3) + j, acc[sm][sn][r]);
+                    visit(APK ? wm0 + 32 * (sm >> 1) + 2 * (4 * b + i) + (sm & 1) : wm0 + sm * 16 + 4 * b + i,
+                          wn0 + sn * 16 + 4 * ((b + r) & 3) + j, acc[sm][sn][r]);
     }
 };
 
@@ -308,5 +342,19 @@ template <typename T, int WTM, int WTN, int BK> struct WaveSel;
 template <int WTM, int WTN, int BK> struct WaveSel<double, WTM, WTN, BK> { using type = WaveF64<WTM, WTN, BK>; };
 template <int WTM, int WTN, int BK> struct WaveSel<float, WTM, WTN, BK> { using type = WaveF32<WTM, WTN, BK>; };
 template <int WTM, int WTN, int BK> struct WaveSel<uint16_t, WTM, WTN, BK> { using type = WaveBF16<WTM, WTN, BK>; };
+
+// LDS slab layout of the single-problem kernel: a rows-contiguous FP64 A
+// (not transposed) goes k-major and is read in row pairs; everything else is
+// [row][k].  STAGE = elements of one A + B stage.
+template <typename T, bool TA, bool TB, int BM, int BN, int BK>
+struct SlabLayout {
+    static constexpr bool KMA = sizeof(T) == 8 && !TA;
+    static constexpr int PITCH = BK + 16 / (int)sizeof(T);
+    static constexpr int SA = KMA ? BK * (BM + 4) : BM * PITCH;
+    static constexpr int STAGE = SA + BN * PITCH;
+    template <int WTM, int WTN>
+    using Wave = std::conditional_t<sizeof(T) == 8, WaveF64<WTM, WTN, BK, KMA ? BM + 4 : 0>,
+                                    typename WaveSel<T, WTM, WTN, BK>::type>;
+};
 
 }  // namespace gemm_amd
