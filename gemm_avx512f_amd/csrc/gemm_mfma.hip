@@ -34,7 +34,7 @@ void gemm_mfma_kernel(GemmParams<T> p) {
     constexpr int WTM = BM / WM, WTN = BN / WN;
     using Lay = SlabLayout<T, TA, TB, BM, BN, BK>;
     using LA = TileLoader<T, BM, BK, NT, TA, VEC, Lay::KMA>;   // op(A) rows = M; k-contiguous iff A is transposed
-    using LB = TileLoader<T, BN, BK, NT, !TB, VEC>;            // op(B) rows = N; k-contiguous iff B is not transposed
+    using LB = TileLoader<T, BN, BK, NT, !TB, VEC, Lay::KMB>;  // op(B) rows = N; k-contiguous iff B is not transposed
     using Wave = typename Lay::template Wave<WTM, WTN>;
     constexpr int STAGE = Lay::STAGE;
     constexpr int BOFF = LA::SIZE;

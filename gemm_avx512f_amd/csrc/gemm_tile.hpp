@@ -147,13 +147,16 @@ struct TileLoader {
 // row pairs: lane q reads rows 2q, 2q+1 of a 32-row band at one k (one
 // 16-byte read) and feeds the first to sub-tile 2p (the band's even rows) and
 // the second to sub-tile 2p+1 (its odd rows), both against the same B
-// fragment; the two k of a k pair are two such reads.
-template <int WTM, int WTN, int BK, int APK = 0>
+// fragment; the two k of a k pair are two such reads.  BPK > 0 does the same
+// for a k-major B slab: column pairs of a 32-column band feed the band's even
+// and odd sub-tiles (the rotation picks the pair).
+template <int WTM, int WTN, int BK, int APK = 0, int BPK = 0>
 struct WaveF64 {
     static constexpr int SM = WTM / 16, SN = WTN / 16;
     static constexpr int PITCH = BK + 2;
     static_assert(BK % 8 == 0, "FP64 slab depth must be a multiple of 8");
     static_assert(APK == 0 || SM % 2 == 0, "row pairs need an even number of A sub-tiles");
+    static_assert(BPK == 0 || SN % 2 == 0, "column pairs need an even number of B sub-tiles");
     double acc[SM][SN][4];
     int aoff[SM], boff[SN][4];
 
@@ -167,7 +170,8 @@ struct WaveF64 {
         for (int sn = 0; sn < SN; ++sn)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                boff[sn][r] = (wn0 + sn * 16 + 4 * ((b + r) & 3) + j) * PITCH + (BK / 4) * g;
+                boff[sn][r] = BPK ? (BK / 4) * g * BPK + wn0 + 32 * (sn >> 1) + 2 * (4 * ((b + r) & 3) + j)
+                                  : (wn0 + sn * 16 + 4 * ((b + r) & 3) + j) * PITCH + (BK / 4) * g;
 #pragma unroll
         for (int sm = 0; sm < SM; ++sm)
 #pragma unroll
@@ -191,10 +195,21 @@ struct WaveF64 {
 #pragma unroll
                 for (int sm = 0; sm < SM; ++sm) av[sm] = *reinterpret_cast<const d2_t *>(As + aoff[sm] + 2 * h);
             }
+            if (BPK) {
+                // bv[2p + e][r] = columns of band p, rotation r, at k = 2h + e
 #pragma unroll
-            for (int sn = 0; sn < SN; ++sn)
+                for (int p = 0; p < SN / 2; ++p)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) bv[sn][r] = *reinterpret_cast<const d2_t *>(Bs + boff[sn][r] + 2 * h);
+                    for (int e = 0; e < 2; ++e)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            bv[2 * p + e][r] = *reinterpret_cast<const d2_t *>(Bs + boff[2 * p][r] + (2 * h + e) * BPK);
+            } else {
+#pragma unroll
+                for (int sn = 0; sn < SN; ++sn)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) bv[sn][r] = *reinterpret_cast<const d2_t *>(Bs + boff[sn][r] + 2 * h);
+            }
 #pragma unroll
             for (int e = 0; e < 2; ++e)
 #pragma unroll
@@ -205,7 +220,8 @@ struct WaveF64 {
                         for (int r = 0; r < 4; ++r) {
                             // k-major: sub-tile sm = 2p + hf takes element hf of the band's read at k = 2h + e
                             const double a = APK ? av[2 * (sm >> 1) + e][sm & 1] : av[sm][e];
-                            acc[sm][sn][r] = mfma_f64_4x4x4(a, bv[sn][r][e], acc[sm][sn][r]);
+                            const double bb = BPK ? bv[2 * (sn >> 1) + e][r][sn & 1] : bv[sn][r][e];
+                            acc[sm][sn][r] = mfma_f64_4x4x4(a, bb, acc[sm][sn][r]);
                         }
             // keep each phase's fragment reads next to its matrix ops: hoisting
             // every phase's reads to the top would need 4x the fragment registers
@@ -224,7 +240,9 @@ struct WaveF64 {
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
                     visit(APK ? wm0 + 32 * (sm >> 1) + 2 * (4 * b + i) + (sm & 1) : wm0 + sm * 16 + 4 * b + i,
-                          wn0 + sn * 16 + 4 * ((b + r) & 3) + j, acc[sm][sn][r]);
+                          BPK ? wn0 + 32 * (sn >> 1) + 2 * (4 * ((b + r) & 3) + j) + (sn & 1)
+                              : wn0 + sn * 16 + 4 * ((b + r) & 3) + j,
+                          acc[sm][sn][r]);
     }
 };
 
@@ -343,17 +361,20 @@ template <int WTM, int WTN, int BK> struct WaveSel<double, WTM, WTN, BK> { using
 template <int WTM, int WTN, int BK> struct WaveSel<float, WTM, WTN, BK> { using type = WaveF32<WTM, WTN, BK>; };
 template <int WTM, int WTN, int BK> struct WaveSel<uint16_t, WTM, WTN, BK> { using type = WaveBF16<WTM, WTN, BK>; };
 
-// LDS slab layout of the single-problem kernel: a rows-contiguous FP64 A
-// (not transposed) goes k-major and is read in row pairs; everything else is
-// [row][k].  STAGE = elements of one A + B stage.
+// LDS slab layout of the single-problem kernel: rows-contiguous FP64
+// operands (A not transposed, B transposed) go k-major and are read in row /
+// column pairs; everything else is [row][k].  STAGE = elements of one A + B
+// stage.
 template <typename T, bool TA, bool TB, int BM, int BN, int BK>
 struct SlabLayout {
     static constexpr bool KMA = sizeof(T) == 8 && !TA;
+    static constexpr bool KMB = sizeof(T) == 8 && TB;
     static constexpr int PITCH = BK + 16 / (int)sizeof(T);
     static constexpr int SA = KMA ? BK * (BM + 4) : BM * PITCH;
-    static constexpr int STAGE = SA + BN * PITCH;
+    static constexpr int SB = KMB ? BK * (BN + 4) : BN * PITCH;
+    static constexpr int STAGE = SA + SB;
     template <int WTM, int WTN>
-    using Wave = std::conditional_t<sizeof(T) == 8, WaveF64<WTM, WTN, BK, KMA ? BM + 4 : 0>,
+    using Wave = std::conditional_t<sizeof(T) == 8, WaveF64<WTM, WTN, BK, KMA ? BM + 4 : 0, KMB ? BN + 4 : 0>,
                                     typename WaveSel<T, WTM, WTN, BK>::type>;
 };
 
