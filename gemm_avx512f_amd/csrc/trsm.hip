@@ -13,9 +13,10 @@
 //   3. recursive halving over the triangle: solve the leading half, update the
 //      trailing half with one large GEMM (the FP64 matrix-core kernel), solve
 //      the trailing half.  A leaf of up to NL rows is one launch of the panel
-//      kernel: each workgroup holds a slab of WC right-hand sides in LDS and
-//      walks the leaf's NB-row sub-blocks, X_q = inv(T_qq) B_q followed by the
-//      update of the leaf's remaining rows, in place.
+//      kernel: each workgroup keeps a slab of 4 (or 8) right-hand sides in
+//      FP64 matrix-core accumulators and walks the leaf's NB-row sub-blocks,
+//      X_q = inv(T_qq) B_q followed by the update of the leaf's remaining
+//      rows, in place.
 // Side R is solved in the same "left" form: X op(A) = B  <=>  op(A)^T X^T = B^T,
 // so the panel kernel and the diagonal inverses see T = op(A)^T and B^T
 // through strides; the trailing updates stay GEMMs on B as stored.
