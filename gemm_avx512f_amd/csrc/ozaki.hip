@@ -54,15 +54,30 @@ __global__ __launch_bounds__(256) void oz_zero_u64_kernel(unsigned long long *p,
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = 0ull;
 }
 
+// All 16 loads of a thread are issued before any is used (entries past the
+// matrix edge read element 0 and are replaced by zero afterwards), so the
+// tile costs one memory latency, not sixteen.  R, K >= 1.
 __device__ __forceinline__ void oz_load_tile(const double *__restrict__ X, int64_t rs, int64_t ks, int R, int K,
                                              int r0, int k0, double (*T)[65]) {
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < 64 * 64; idx += 256) {
-        int rr, kk;
+    auto coord = [&](int i, int &rr, int &kk) {
+        const int idx = tid + 256 * i;
         if (rs == 1) { rr = idx & 63; kk = idx >> 6; }   // rows contiguous in memory
         else { kk = idx & 63; rr = idx >> 6; }           // k contiguous
+    };
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        int rr, kk;
+        coord(i, rr, kk);
         const int r = r0 + rr, k = k0 + kk;
-        T[rr][kk] = (r < R && k < K) ? X[(int64_t)r * rs + (int64_t)k * ks] : 0.0;
+        v[i] = X[(r < R && k < K) ? (int64_t)r * rs + (int64_t)k * ks : 0];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        int rr, kk;
+        coord(i, rr, kk);
+        T[rr][kk] = (r0 + rr < R && k0 + kk < K) ? v[i] : 0.0;
     }
 }
 
