@@ -680,9 +680,9 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
                 } else {
                     if (st + 1 < total) gload(st + 1);
                 }
-                if constexpr (DIRECT && TM * TN <= 8) {
+                if constexpr (DIRECT) {
                     // fragments of k-step ks + 1 are read while the matrix
-                    // instructions of k-step ks run (register room permitting)
+                    // instructions of k-step ks run
                     i4_t av[2][TM], bv[2][TN];
                     auto frag = [&](int ks, i4_t *a, i4_t *b) {
                         const int ko = ((2 * ks + g) ^ qx) << 4;
@@ -1004,7 +1004,6 @@ int launch_dgemm_ozaki_crt(bool ta, bool tb, int m, int n, int k, double alpha, 
     if (v == 1) return launch_crt_variant<2, 4, 4, 2, 128>(p, s);   // register staging
     if (v == 2) return launch_crt_variant<2, 4, 4, 2, 64>(p, s);    // register staging, 64-deep stages
     if (v == 4) return launch_crt_variant<2, 4, 4, 2, 64, true, 4>(p, s);   // direct, 4 stages of 64 bytes
-    if (v == 5) return launch_crt_variant<2, 2, 4, 4, 128, true>(p, s);     // direct, 4 waves of 128x128
     return launch_crt_variant<2, 4, 4, 2, 128, true>(p, s);
 }
 
