@@ -115,6 +115,35 @@ struct TileLoader {
         }
     }
 
+    // Whole-chunk variant of load() through a buffer resource over the
+    // operand (num_records 2^31 - 1 bytes), for operands whose 16-byte chunks
+    // lie either wholly inside or wholly outside (rlim, klim) (the chunked
+    // dimension a multiple of VW, rows 16-byte aligned) and whose extent is
+    // below 2 GiB.  `off` is the element offset of the slab's origin.  Every
+    // chunk is one unconditional load at a 32-bit byte offset; an out-of-range
+    // chunk is given offset 2^31, past num_records, which the buffer unit
+    // returns as zeros.  The loads carry no branches, so a second load set
+    // issued after this one does not make the store of this one wait for it.
+    __device__ __forceinline__ void load_buf(__amdgpu_buffer_rsrc_t rs, int64_t off, int64_t ld, int rlim, int klim,
+                                             int tid) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            int r, k;
+            coords(tid + i * NT, r, k);
+            const bool ok = r < rlim && k < klim;
+            const int64_t e0 = off + (KCONTIG ? (int64_t)r * ld + k : r + (int64_t)k * ld);
+            const int bo = ok ? (int)(e0 * (int64_t)sizeof(T)) : (int)0x80000000u;
+            if constexpr (KGATHER) {
+#pragma unroll
+                for (int e = 0; e < VW; ++e)
+                    reg[i][e] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(
+                                                          rs, bo, (int)(e * ld * (int64_t)sizeof(T)), 0));
+            } else {
+                reg[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(rs, bo, 0, 0));
+            }
+        }
+    }
+
     __device__ __forceinline__ void store(T *lds, int tid) const {
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
@@ -131,6 +160,12 @@ struct TileLoader {
         }
     }
 };
+
+// Buffer resource over memory at p for the raw buffer loads above: stride 0,
+// num_records 2^31 - 1 bytes, the gfx9 data-format word (32-bit elements).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t operand_rsrc(const void *p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, 0x7fffffff, 0x00020000);
+}
 
 // ---------------------------------------------------------------- wave tiles
 // FP64: v_mfma_f64_4x4x4_4b.  A 16x16 sub-tile is four instructions: all use

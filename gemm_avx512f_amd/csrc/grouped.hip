@@ -24,6 +24,7 @@ namespace gemm_amd {
 
 struct GroupDesc {
     int m, n, k, ta, tb, tiles_m, tiles_n, vec;   // vec: A, B 16-byte aligned with ld a multiple of 16 bytes
+    int chunked;                                  // vec, and m, n, k multiples of the 16-byte chunk
     int64_t lda, ldb, ldc, tile_base;
     const void *A, *B;
     void *C;
@@ -31,8 +32,15 @@ struct GroupDesc {
 };
 
 // One BM x BN output tile of problem d; TI = operand element type, TO = C's.
-template <typename TI, typename TO, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN>
-__device__ void grouped_tile(const GroupDesc &d, int tm, int tn, unsigned char *smem_raw) {   // d: in global memory
+// CHUNKED problems (every dimension a multiple of the 16-byte chunk, aligned
+// operands) load whole chunks without branches and fetch slabs two ahead:
+// while slab kt is multiplied out of LDS, slab kt+1 waits in one register set
+// for its LDS store and slab kt+2 is in flight into the other, so each load
+// has two slabs of matrix work to arrive in.  The others take load()'s
+// range-checked element path one slab ahead.
+template <typename TI, typename TO, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, bool CHUNKED>
+__device__ __forceinline__ void grouped_tile(const GroupDesc &d, int tm, int tn, unsigned char *smem_raw,
+                                             int tid) {   // d: in global memory
     constexpr int NT = WM * WN * 64;
     constexpr int WTM = BM / WM, WTN = BN / WN;
     // vector loads where the problem's operands allow them (a run-time choice:
@@ -45,7 +53,7 @@ __device__ void grouped_tile(const GroupDesc &d, int tm, int tn, unsigned char *
     TI *smem = reinterpret_cast<TI *>(smem_raw);
     const TI *A = static_cast<const TI *>(d.A);
     const TI *B = static_cast<const TI *>(d.B);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lane = tid & 63, wave = tid >> 6;
     const int wm0 = (wave % WM) * WTM, wn0 = (wave / WM) * WTN;
     const int m0 = tm * BM, n0 = tn * BN;
     const int mlim = d.m - m0, nlim = d.n - n0;
@@ -53,34 +61,72 @@ __device__ void grouped_tile(const GroupDesc &d, int tm, int tn, unsigned char *
     const TI *bbase = TB ? B + n0 : B + (int64_t)n0 * d.ldb;
     const int64_t astep = TA ? BK : (int64_t)BK * d.lda;
     const int64_t bstep = TB ? (int64_t)BK * d.ldb : BK;
-    LA la;
-    LB lb;
-    la.vec = lb.vec = d.vec != 0;
     Wave w;
     w.init(lane, wm0, wn0);
     const int nk = (d.k + BK - 1) / BK;
-    if (nk > 0) {
-        la.load(abase, d.lda, mlim, d.k, tid);
-        lb.load(bbase, d.ldb, nlim, d.k, tid);
-        la.store(smem, tid);
-        lb.store(smem + BM * PITCH, tid);
-    }
-    __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-        TI *cur = smem + (kt & 1) * STAGE;
-        TI *nxt = smem + ((kt + 1) & 1) * STAGE;
-        const bool more = kt + 1 < nk;
-        if (more) {
-            const int krem = d.k - (kt + 1) * BK;
-            la.load(abase + (kt + 1) * astep, d.lda, mlim, krem, tid);
-            lb.load(bbase + (kt + 1) * bstep, d.ldb, nlim, krem, tid);
-        }
-        w.compute(cur, cur + BM * PITCH);
-        if (more) {
-            la.store(nxt, tid);
-            lb.store(nxt + BM * PITCH, tid);
+    if constexpr (CHUNKED) {
+        LA la0, la1;
+        LB lb0, lb1;
+        const __amdgpu_buffer_rsrc_t ra = operand_rsrc(A), rb = operand_rsrc(B);
+        const int64_t aoff = abase - A, boff = bbase - B;
+        // slab kt into one register set; past the last slab every chunk reads as zero
+        auto fetch = [&](LA &la, LB &lb, int kt) {
+            const int krem = d.k - kt * BK;
+            la.load_buf(ra, aoff + kt * astep, d.lda, mlim, krem, tid);
+            lb.load_buf(rb, boff + kt * bstep, d.ldb, nlim, krem, tid);
+        };
+        // multiply slab kt; (sa, sb) hold slab kt+1, (ia, ib) receive slab kt+2
+        auto step = [&](int kt, const LA &sa, const LB &sb, LA &ia, LB &ib) {
+            TI *cur = smem + (kt & 1) * STAGE;
+            TI *nxt = smem + ((kt + 1) & 1) * STAGE;
+            fetch(ia, ib, kt + 2);
+            w.compute(cur, cur + BM * PITCH);
+            if (kt + 1 < nk) {
+                sa.store(nxt, tid);
+                sb.store(nxt + BM * PITCH, tid);
+            }
+            __syncthreads();
+        };
+        if (nk > 0) {
+            fetch(la0, lb0, 0);
+            fetch(la1, lb1, 1);
+            la0.store(smem, tid);
+            lb0.store(smem + BM * PITCH, tid);
         }
         __syncthreads();
+        for (int kt = 0; kt < nk; kt += 2) {
+            step(kt, la1, lb1, la0, lb0);
+            if (kt + 1 < nk) step(kt + 1, la0, lb0, la1, lb1);
+        }
+    } else {
+        // range-checked element path, one slab ahead (its branches would make a
+        // second register set wait with the first)
+        LA la;
+        LB lb;
+        la.vec = lb.vec = d.vec != 0;
+        if (nk > 0) {
+            la.load(abase, d.lda, mlim, d.k, tid);
+            lb.load(bbase, d.ldb, nlim, d.k, tid);
+            la.store(smem, tid);
+            lb.store(smem + BM * PITCH, tid);
+        }
+        __syncthreads();
+        for (int kt = 0; kt < nk; ++kt) {
+            TI *cur = smem + (kt & 1) * STAGE;
+            TI *nxt = smem + ((kt + 1) & 1) * STAGE;
+            const bool more = kt + 1 < nk;
+            if (more) {
+                const int krem = d.k - (kt + 1) * BK;
+                la.load(abase + (kt + 1) * astep, d.lda, mlim, krem, tid);
+                lb.load(bbase + (kt + 1) * bstep, d.ldb, nlim, krem, tid);
+            }
+            w.compute(cur, cur + BM * PITCH);
+            if (more) {
+                la.store(nxt, tid);
+                lb.store(nxt + BM * PITCH, tid);
+            }
+            __syncthreads();
+        }
     }
     TO *cbase = static_cast<TO *>(d.C) + m0 + (int64_t)n0 * d.ldc;
     const TO alpha = (TO)d.alpha, beta = (TO)d.beta;
@@ -106,11 +152,20 @@ __global__ __launch_bounds__(WM *WN * 64) void grouped_gemm_kernel(const GroupDe
         const GroupDesc &d = descs[lo];   // read in place (uniform scalar loads), never copied to private memory
         const int local = (int)(t - d.tile_base);
         const int tm = local % d.tiles_m, tn = local / d.tiles_m;
-        switch (d.ta * 2 + d.tb) {
-        case 0: grouped_tile<TI, TO, false, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
-        case 1: grouped_tile<TI, TO, false, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
-        case 2: grouped_tile<TI, TO, true, false, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
-        default: grouped_tile<TI, TO, true, true, BM, BN, BK, WM, WN>(d, tm, tn, smem); break;
+        // the thread index passes through an empty asm each tile, so the eight
+        // tile variants' LDS addresses are formed inside their own code paths
+        // instead of being hoisted out of this loop and held live in registers
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        switch (d.ta * 4 + d.tb * 2 + d.chunked) {
+        case 0: grouped_tile<TI, TO, false, false, BM, BN, BK, WM, WN, false>(d, tm, tn, smem, tid); break;
+        case 1: grouped_tile<TI, TO, false, false, BM, BN, BK, WM, WN, true>(d, tm, tn, smem, tid); break;
+        case 2: grouped_tile<TI, TO, false, true, BM, BN, BK, WM, WN, false>(d, tm, tn, smem, tid); break;
+        case 3: grouped_tile<TI, TO, false, true, BM, BN, BK, WM, WN, true>(d, tm, tn, smem, tid); break;
+        case 4: grouped_tile<TI, TO, true, false, BM, BN, BK, WM, WN, false>(d, tm, tn, smem, tid); break;
+        case 5: grouped_tile<TI, TO, true, false, BM, BN, BK, WM, WN, true>(d, tm, tn, smem, tid); break;
+        case 6: grouped_tile<TI, TO, true, true, BM, BN, BK, WM, WN, false>(d, tm, tn, smem, tid); break;
+        default: grouped_tile<TI, TO, true, true, BM, BN, BK, WM, WN, true>(d, tm, tn, smem, tid); break;
         }
         __syncthreads();   // the next tile reuses the LDS stages
     }
@@ -193,6 +248,12 @@ void *grouped_plan_create(int dtype, int count, const char *ta, const char *tb, 
             const int vw = 16 / eb;
             d.vec = ((uintptr_t)A[p] % 16 == 0) && ((uintptr_t)B[p] % 16 == 0) && (lda[p] % vw == 0) &&
                     (ldb[p] % vw == 0);
+            // extents in bytes, for the 32-bit buffer offsets of the chunked path
+            const int64_t ar = d.ta ? k[p] : m[p], ac = d.ta ? m[p] : k[p];
+            const int64_t br = d.tb ? n[p] : k[p], bc = d.tb ? k[p] : n[p];
+            const int64_t ea = ((int64_t)lda[p] * (ac - 1) + ar) * eb, ebb = ((int64_t)ldb[p] * (bc - 1) + br) * eb;
+            d.chunked = d.vec && m[p] % vw == 0 && n[p] % vw == 0 && k[p] % vw == 0 && ea < (1ll << 31) - 64 &&
+                        ebb < (1ll << 31) - 64;
         }
         d.lda = lda[p];
         d.ldb = ldb[p];

@@ -98,9 +98,17 @@ def oracle_check(dtype, p, out):
     assert err <= 4 * (k + 2) * u, (p["transa"], p["transb"], m, n, k, err)
 
 
+# "ragged": arbitrary sizes and padded leading dimensions (range-checked element
+# loads); "chunked": sizes and leading dimensions multiples of 8 (whole 16-byte
+# chunk loads through buffer resources, two slabs ahead), still ragged against
+# the 64 x 64 tiles and the slab depth
+SHAPES = {"ragged": dict(mult=1, pad_max=3), "chunked": dict(mult=8, pad_max=0)}
+
+
+@pytest.mark.parametrize("shapes", sorted(SHAPES))
 @pytest.mark.parametrize("dtype", [GROUPED_F32, GROUPED_BF16, GROUPED_F64], ids=["f32", "bf16", "f64"])
-def test_grouped_vs_oracle(dtype):
-    P = upload(random_problems(dtype, 40, seed=11 + dtype, lo=1, hi=300, zero_dims=True))
+def test_grouped_vs_oracle(dtype, shapes):
+    P = upload(random_problems(dtype, 40, seed=11 + dtype, lo=1, hi=300, zero_dims=True, **SHAPES[shapes]))
     g = GroupedGemm(dtype, P)
     bm, bn = GROUP_TILE[dtype]
     assert g.tiles == sum(-(-p["m"] // bm) * -(-p["n"] // bn) for p in P)
@@ -111,9 +119,10 @@ def test_grouped_vs_oracle(dtype):
     g.destroy()
 
 
+@pytest.mark.parametrize("shapes", sorted(SHAPES))
 @pytest.mark.parametrize("dtype", [GROUPED_F32, GROUPED_F64], ids=["f32", "f64"])
-def test_grouped_bitwise_equals_single_gemm(dtype):
-    P = upload(random_problems(dtype, 30, seed=21 + dtype, lo=8, hi=500))
+def test_grouped_bitwise_equals_single_gemm(dtype, shapes):
+    P = upload(random_problems(dtype, 30, seed=21 + dtype, lo=8, hi=500, **SHAPES[shapes]))
     g = GroupedGemm(dtype, P)
     g.run()
     single = blas.sgemm if dtype == GROUPED_F32 else blas.dgemm
