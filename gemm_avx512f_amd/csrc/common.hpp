@@ -39,12 +39,33 @@ __device__ __forceinline__ f16_t mfma_f32_32x32x2(float a, float b, f16_t c) {
 // The GEMM epilogue alpha*v + beta*c, spelled with one explicit fused
 // multiply-add so that every kernel rounds it identically (beta == 0 does not
 // read c).
-template <typename T>
-__device__ __forceinline__ T axpby(T alpha, T v, T beta, const T *c) {
-    return beta == T(0) ? alpha * v : __builtin_fma(beta, *c, alpha * v);
+// c points at C's element in any address space.
+template <typename T, typename P>
+__device__ __forceinline__ T axpby(T alpha, T v, T beta, P c) {
+    if (beta == T(0)) return alpha * v;
+    const T cv = *c;
+    if constexpr (sizeof(T) == 4) return __builtin_fmaf(beta, cv, alpha * v);
+    else return __builtin_fma(beta, cv, alpha * v);
 }
-__device__ __forceinline__ float axpby(float alpha, float v, float beta, const float *c) {
-    return beta == 0.0f ? alpha * v : __builtin_fmaf(beta, *c, alpha * v);
+
+// The same pointer in the global address space.  Accesses through pointers
+// read from memory (a descriptor table) otherwise compile to flat
+// instructions, which count against both the vector-memory and the LDS
+// counters and so turn the counted waits of a load pipeline into full drains.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *as_global(T *p) {
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+// Workgroup barrier that orders LDS only.  __syncthreads() fences every
+// address space, so it drains the wave's outstanding global loads as well
+// (s_waitcnt vmcnt(0) before s_barrier); fenced on LDS alone, the barrier
+// waits for the wave's LDS accesses (lgkmcnt) and leaves loads that are
+// prefetching later slabs in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // Map a linear workgroup id to a tile id so that each of the 8 XCDs (which

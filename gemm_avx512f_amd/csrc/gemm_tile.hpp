@@ -86,13 +86,18 @@ struct TileLoader {
         }
     }
 
+    // Range-checked slab (edges, unaligned operands).  The operand is read
+    // through a global-address-space pointer: pointers taken from a
+    // descriptor table would otherwise compile to flat loads.
     __device__ __forceinline__ void load(const T *__restrict__ base, int64_t ld, int rlim, int klim,
                                          int tid) {
+        using GP = const __attribute__((address_space(1))) T *;
+        const GP gbase = (GP)base;
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             int r, k;
             coords(tid + i * NT, r, k);
-            const T *src = KCONTIG ? base + (int64_t)r * ld + k : base + r + (int64_t)k * ld;
+            const GP src = KCONTIG ? gbase + (int64_t)r * ld + k : gbase + r + (int64_t)k * ld;
             if (KGATHER) {
                 V t;
 #pragma unroll
@@ -102,7 +107,7 @@ struct TileLoader {
             }
             const bool full = KCONTIG ? (r < rlim && k + VW <= klim) : (r + VW <= rlim && k < klim);
             if (VEC && vec && full) {
-                reg[i] = *reinterpret_cast<const V *>(src);
+                reg[i] = *reinterpret_cast<const __attribute__((address_space(1))) V *>(src);
             } else {
                 V t;
 #pragma unroll
@@ -115,15 +120,15 @@ struct TileLoader {
         }
     }
 
-    // Whole-chunk variant of load() through a buffer resource over the
-    // operand (num_records 2^31 - 1 bytes), for operands whose 16-byte chunks
-    // lie either wholly inside or wholly outside (rlim, klim) (the chunked
+    // Whole-chunk loads through a buffer resource over the operand
+    // (num_records 2^31 - 1 bytes), for operands whose 16-byte chunks lie
+    // either wholly inside or wholly outside (rlim, klim) (the chunked
     // dimension a multiple of VW, rows 16-byte aligned) and whose extent is
-    // below 2 GiB.  `off` is the element offset of the slab's origin.  Every
-    // chunk is one unconditional load at a 32-bit byte offset; an out-of-range
-    // chunk is given offset 2^31, past num_records, which the buffer unit
-    // returns as zeros.  The loads carry no branches, so a second load set
-    // issued after this one does not make the store of this one wait for it.
+    // below 2 GiB.  Every chunk is one unconditional load at a 32-bit byte
+    // offset; an out-of-range chunk is given offset 2^31, past num_records,
+    // which the buffer unit returns as zeros.
+    //
+    // load_buf: the slab at element offset `off`, range-checked in rows and k.
     __device__ __forceinline__ void load_buf(__amdgpu_buffer_rsrc_t rs, int64_t off, int64_t ld, int rlim, int klim,
                                              int tid) {
 #pragma unroll
@@ -132,15 +137,41 @@ struct TileLoader {
             coords(tid + i * NT, r, k);
             const bool ok = r < rlim && k < klim;
             const int64_t e0 = off + (KCONTIG ? (int64_t)r * ld + k : r + (int64_t)k * ld);
-            const int bo = ok ? (int)(e0 * (int64_t)sizeof(T)) : (int)0x80000000u;
-            if constexpr (KGATHER) {
+            load_chunk(rs, i, ok ? (int)(e0 * (int64_t)sizeof(T)) : (int)0x80000000u, 0, ld);
+        }
+    }
+
+    // For the k-loop over full slabs: the per-chunk byte offsets of the first
+    // slab (rows past rlim at 2^31) are computed once per tile, and each slab
+    // adds a wave-uniform byte offset `soff` (the instruction's scalar
+    // offset), so the loop rewrites no address registers.  (A rewritten
+    // address register that is still the source of an outstanding load makes
+    // the compiler wait for that load.)
+    struct Offsets {
+        int v[PER];
+    };
+    __device__ __forceinline__ static void chunk_offsets(Offsets &o, int64_t off, int64_t ld, int rlim, int tid) {
 #pragma unroll
-                for (int e = 0; e < VW; ++e)
-                    reg[i][e] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(
-                                                          rs, bo, (int)(e * ld * (int64_t)sizeof(T)), 0));
-            } else {
-                reg[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(rs, bo, 0, 0));
-            }
+        for (int i = 0; i < PER; ++i) {
+            int r, k;
+            coords(tid + i * NT, r, k);
+            const int64_t e0 = off + (KCONTIG ? (int64_t)r * ld + k : r + (int64_t)k * ld);
+            o.v[i] = r < rlim ? (int)(e0 * (int64_t)sizeof(T)) : (int)0x80000000u;
+        }
+    }
+    __device__ __forceinline__ void load_slab(__amdgpu_buffer_rsrc_t rs, const Offsets &o, int soff, int64_t ld) {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) load_chunk(rs, i, o.v[i], soff, ld);
+    }
+
+    __device__ __forceinline__ void load_chunk(__amdgpu_buffer_rsrc_t rs, int i, int voff, int soff, int64_t ld) {
+        if constexpr (KGATHER) {
+#pragma unroll
+            for (int e = 0; e < VW; ++e)
+                reg[i][e] = __builtin_bit_cast(
+                    T, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff + (int)(e * ld * (int64_t)sizeof(T)), 0));
+        } else {
+            reg[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0));
         }
     }
 

@@ -69,34 +69,61 @@ __device__ __forceinline__ void grouped_tile(const GroupDesc &d, int tm, int tn,
         LB lb0, lb1;
         const __amdgpu_buffer_rsrc_t ra = operand_rsrc(A), rb = operand_rsrc(B);
         const int64_t aoff = abase - A, boff = bbase - B;
-        // slab kt into one register set; past the last slab every chunk reads as zero
-        auto fetch = [&](LA &la, LB &lb, int kt) {
-            const int krem = d.k - kt * BK;
-            la.load_buf(ra, aoff + kt * astep, d.lda, mlim, krem, tid);
-            lb.load_buf(rb, boff + kt * bstep, d.ldb, nlim, krem, tid);
-        };
-        // multiply slab kt; (sa, sb) hold slab kt+1, (ia, ib) receive slab kt+2
-        auto step = [&](int kt, const LA &sa, const LB &sb, LA &ia, LB &ib) {
-            TI *cur = smem + (kt & 1) * STAGE;
-            TI *nxt = smem + ((kt + 1) & 1) * STAGE;
-            fetch(ia, ib, kt + 2);
-            w.compute(cur, cur + BM * PITCH);
-            if (kt + 1 < nk) {
+        const int nf = d.k / BK;   // full slabs; a partial last one follows them
+        if (nf > 0) {
+            typename LA::Offsets oa;
+            typename LB::Offsets ob;
+            LA::chunk_offsets(oa, aoff, d.lda, mlim, tid);
+            LB::chunk_offsets(ob, boff, d.ldb, nlim, tid);
+            const int sa_step = (int)(astep * (int64_t)sizeof(TI)), sb_step = (int)(bstep * (int64_t)sizeof(TI));
+            // full slab kt (clamped to the last one: the prefetch past the end
+            // re-reads it into the idle register set)
+            auto fetch = [&](LA &la, LB &lb, int kt) {
+                const int ks = kt < nf ? kt : nf - 1;
+                la.load_slab(ra, oa, ks * sa_step, d.lda);
+                lb.load_slab(rb, ob, ks * sb_step, d.ldb);
+            };
+            // multiply slab kt; (sa, sb) hold slab kt+1, (ia, ib) receive slab kt+2
+            auto step = [&](int kt, const LA &sa, const LB &sb, LA &ia, LB &ib) {
+                TI *cur = smem + (kt & 1) * STAGE;
+                TI *nxt = smem + ((kt + 1) & 1) * STAGE;
+                fetch(ia, ib, kt + 2);
+                __builtin_amdgcn_sched_barrier(0);   // issue the loads before the slab's work
+                w.compute(cur, cur + BM * PITCH);
+                // unconditional (after the last slab it fills the idle stage):
+                // a skipped store would leave this set's loads pending on one
+                // path and cost exact wait counts everywhere after it
                 sa.store(nxt, tid);
                 sb.store(nxt + BM * PITCH, tid);
-            }
-            __syncthreads();
-        };
-        if (nk > 0) {
+                lds_barrier();   // slab kt+2 stays in flight
+            };
             fetch(la0, lb0, 0);
             fetch(la1, lb1, 1);
             la0.store(smem, tid);
             lb0.store(smem + BM * PITCH, tid);
+            lds_barrier();
+            // one exit, at the loop test: every path round the loop runs both
+            // steps and carries exactly one set in flight (la1/lb1), so the
+            // wait counts stay exact (an exit between the steps would merge a
+            // path with la0/lb0 still in flight into the back edge)
+            int kt = 0;
+            for (; kt + 1 < nf; kt += 2) {
+                step(kt, la1, lb1, la0, lb0);
+                step(kt + 1, la0, lb0, la1, lb1);
+            }
+            if (kt < nf) step(kt, la1, lb1, la0, lb0);
         }
-        __syncthreads();
-        for (int kt = 0; kt < nk; kt += 2) {
-            step(kt, la1, lb1, la0, lb0);
-            if (kt + 1 < nk) step(kt + 1, la0, lb0, la1, lb1);
+        if (nf < nk) {
+            // partial last slab, range-checked in k, into the stage after the
+            // last full one (idle since the barrier that ended that slab)
+            TI *st = smem + (nf & 1) * STAGE;
+            const int krem = d.k - nf * BK;
+            la0.load_buf(ra, aoff + nf * astep, d.lda, mlim, krem, tid);
+            lb0.load_buf(rb, boff + nf * bstep, d.ldb, nlim, krem, tid);
+            la0.store(st, tid);
+            lb0.store(st + BM * PITCH, tid);
+            lds_barrier();
+            w.compute(st, st + BM * PITCH);
         }
     } else {
         // range-checked element path, one slab ahead (its branches would make a
@@ -128,14 +155,29 @@ __device__ __forceinline__ void grouped_tile(const GroupDesc &d, int tm, int tn,
             __syncthreads();
         }
     }
-    TO *cbase = static_cast<TO *>(d.C) + m0 + (int64_t)n0 * d.ldc;
+    auto cbase = as_global(static_cast<TO *>(d.C)) + m0 + (int64_t)n0 * d.ldc;
     const TO alpha = (TO)d.alpha, beta = (TO)d.beta;
     w.for_each(lane, wm0, wn0, [&](int r, int c, TO v) {
         if (r < mlim && c < nlim) {
-            TO *dst = cbase + r + (int64_t)c * d.ldc;
+            auto dst = cbase + r + (int64_t)c * d.ldc;
             *dst = axpby(alpha, v, beta, dst);
         }
     });
+}
+
+// One tile of shape BM x BN (slab depth BK) in the variant the problem needs.
+template <typename TI, typename TO, int BM, int BN, int BK, int WM, int WN>
+__device__ __forceinline__ void grouped_dispatch(const GroupDesc &d, int tm, int tn, unsigned char *smem, int tid) {
+    switch (d.ta * 4 + d.tb * 2 + d.chunked) {
+    case 0: grouped_tile<TI, TO, false, false, BM, BN, BK, WM, WN, false>(d, tm, tn, smem, tid); break;
+    case 1: grouped_tile<TI, TO, false, false, BM, BN, BK, WM, WN, true>(d, tm, tn, smem, tid); break;
+    case 2: grouped_tile<TI, TO, false, true, BM, BN, BK, WM, WN, false>(d, tm, tn, smem, tid); break;
+    case 3: grouped_tile<TI, TO, false, true, BM, BN, BK, WM, WN, true>(d, tm, tn, smem, tid); break;
+    case 4: grouped_tile<TI, TO, true, false, BM, BN, BK, WM, WN, false>(d, tm, tn, smem, tid); break;
+    case 5: grouped_tile<TI, TO, true, false, BM, BN, BK, WM, WN, true>(d, tm, tn, smem, tid); break;
+    case 6: grouped_tile<TI, TO, true, true, BM, BN, BK, WM, WN, false>(d, tm, tn, smem, tid); break;
+    default: grouped_tile<TI, TO, true, true, BM, BN, BK, WM, WN, true>(d, tm, tn, smem, tid); break;
+    }
 }
 
 template <typename TI, typename TO, int BM, int BN, int BK, int WM, int WN>
@@ -152,22 +194,13 @@ __global__ __launch_bounds__(WM *WN * 64) void grouped_gemm_kernel(const GroupDe
         const GroupDesc &d = descs[lo];   // read in place (uniform scalar loads), never copied to private memory
         const int local = (int)(t - d.tile_base);
         const int tm = local % d.tiles_m, tn = local / d.tiles_m;
-        // the thread index passes through an empty asm each tile, so the eight
-        // tile variants' LDS addresses are formed inside their own code paths
+        // the thread index passes through an empty asm each tile, so the tile
+        // variants' LDS addresses are formed inside their own code paths
         // instead of being hoisted out of this loop and held live in registers
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        switch (d.ta * 4 + d.tb * 2 + d.chunked) {
-        case 0: grouped_tile<TI, TO, false, false, BM, BN, BK, WM, WN, false>(d, tm, tn, smem, tid); break;
-        case 1: grouped_tile<TI, TO, false, false, BM, BN, BK, WM, WN, true>(d, tm, tn, smem, tid); break;
-        case 2: grouped_tile<TI, TO, false, true, BM, BN, BK, WM, WN, false>(d, tm, tn, smem, tid); break;
-        case 3: grouped_tile<TI, TO, false, true, BM, BN, BK, WM, WN, true>(d, tm, tn, smem, tid); break;
-        case 4: grouped_tile<TI, TO, true, false, BM, BN, BK, WM, WN, false>(d, tm, tn, smem, tid); break;
-        case 5: grouped_tile<TI, TO, true, false, BM, BN, BK, WM, WN, true>(d, tm, tn, smem, tid); break;
-        case 6: grouped_tile<TI, TO, true, true, BM, BN, BK, WM, WN, false>(d, tm, tn, smem, tid); break;
-        default: grouped_tile<TI, TO, true, true, BM, BN, BK, WM, WN, true>(d, tm, tn, smem, tid); break;
-        }
-        __syncthreads();   // the next tile reuses the LDS stages
+        grouped_dispatch<TI, TO, BM, BN, BK, WM, WN>(d, tm, tn, smem, tid);
+        lds_barrier();   // the next tile reuses the LDS stages
     }
 }
 
@@ -185,11 +218,15 @@ struct GroupedPlan {
 
 // Tile configuration per dtype: 64x64 tiles of four 32x32 waves.  The grouped
 // problems (tens to a few thousand per dimension) leave large tiles short of
-// work: 128x128 tiles measured 26 / 63 TFLOP/s (FP32 / bf16) on 512 problems
-// of 32..2048 against 44 / 117 for these 64x64 ones.  bf16 takes 64-deep slabs
-// (77 TFLOP/s with 32-deep ones) so a stage holds more matrix work per barrier.  FP32 and FP64 use the single-problem
-// kernel's 64x64 tile and slab depth, so the k order (and the result) is the
-// same bit for bit.
+// work: with the register-staged loads of the first version, 128x128 tiles
+// measured 26 / 63 TFLOP/s (FP32 / bf16) on 512 problems of 32..2048 against
+// 44 / 117 for these 64x64 ones; with the chunked two-ahead pipeline, bf16
+// problems of M, N >= 128 on 128x128 tiles (32-deep slabs) measured 111 against
+// 128 TFLOP/s on the same mix (their transposing LDS stores conflict more).
+// bf16 takes 64-deep slabs (77 TFLOP/s with 32-deep ones in the first
+// version) so a stage holds more matrix work per barrier.  FP32 and FP64 use
+// the single-problem kernel's 64x64 tile and slab depth, so the k order (and
+// the result) is the same bit for bit.
 template <typename TI, typename TO>
 struct GroupedCfg;
 template <> struct GroupedCfg<float, float> { static constexpr int BM = 64, BN = 64, BK = 32, WM = 2, WN = 2; };

@@ -27,7 +27,14 @@ from tests.util import opmat
 pytestmark = pytest.mark.gpu
 
 # the grouped kernel's tile per dtype and the single-problem tile with the same shape and slab depth
-GROUP_TILE = {GROUPED_F32: (64, 64), GROUPED_BF16: (64, 64), GROUPED_F64: (64, 64)}
+GROUP_TILE = {GROUPED_F32: 64, GROUPED_BF16: 64, GROUPED_F64: 64}
+
+
+def expected_tiles(dtype, problems):
+    t = GROUP_TILE[dtype]
+    return sum(-(-p["m"] // t) * -(-p["n"] // t) for p in problems)
+
+
 SAME_SINGLE_TILE = {GROUPED_F32: 1, GROUPED_F64: 1}   # TILE_64x64
 
 
@@ -110,8 +117,7 @@ SHAPES = {"ragged": dict(mult=1, pad_max=3), "chunked": dict(mult=8, pad_max=0)}
 def test_grouped_vs_oracle(dtype, shapes):
     P = upload(random_problems(dtype, 40, seed=11 + dtype, lo=1, hi=300, zero_dims=True, **SHAPES[shapes]))
     g = GroupedGemm(dtype, P)
-    bm, bn = GROUP_TILE[dtype]
-    assert g.tiles == sum(-(-p["m"] // bm) * -(-p["n"] // bn) for p in P)
+    assert g.tiles == expected_tiles(dtype, P)
     g.run()
     synchronize()
     for p in P:
