@@ -85,10 +85,17 @@ int gemm_amd_zgemm3m_batched(int count, const char *transa, const char *transb, 
     if (count == 0) return 0;
     if (int st = init_status()) return st;
     hipStream_t s = (hipStream_t)stream;
-    // scratch: per problem 3 planes of op(A), op(B) and the three real products
+    // scratch: per problem 3 planes of op(A), op(B) and the three real products,
+    // each dimension padded to even with zeros, so that every real product has
+    // whole 16-byte chunks and aligned planes (the grouped kernel's chunked
+    // path).  The padded k adds exact zero products; padded rows and columns
+    // of the products are not combined.
+    auto even = [](int x) { return x + (x & 1); };
     std::vector<size_t> off(count + 1, 0);
-    for (int p = 0; p < count; ++p)
-        off[p + 1] = off[p] + 3 * ((size_t)m[p] * k[p] + (size_t)k[p] * n[p] + (size_t)m[p] * n[p]);
+    for (int p = 0; p < count; ++p) {
+        const size_t mp = even(m[p]), np = even(n[p]), kp = even(k[p]);
+        off[p + 1] = off[p] + 3 * (mp * kp + kp * np + mp * np);
+    }
     int st = 0;
     double *ws = (double *)device_workspace(WS_COMPLEX, off[count] * sizeof(double) + 256, &st);
     if (st) return st;
@@ -98,25 +105,27 @@ int gemm_amd_zgemm3m_batched(int count, const char *transa, const char *transb, 
     std::vector<const void *> A3, B3;
     std::vector<void *> C3;
     for (int p = 0; p < count; ++p) {
+        const int mp = even(m[p]), np = even(n[p]), kp = even(k[p]);
         double *pa = ws + off[p];
-        double *pb = pa + 3 * (size_t)m[p] * k[p];
-        double *pt = pb + 3 * (size_t)k[p] * n[p];
-        int e = launch_gemm3m_split_d(transa[p], transb[p], m[p], n[p], k[p], a[p], lda[p], b[p], ldb[p], pa, pb, s);
+        double *pb = pa + 3 * (size_t)mp * kp;
+        double *pt = pb + 3 * (size_t)kp * np;
+        int e = launch_gemm3m_split_d(transa[p], transb[p], m[p], n[p], k[p], mp, np, kp, a[p], lda[p], b[p], ldb[p],
+                                      pa, pb, s);
         if (e) return hip_status((hipError_t)e, "ZGEMM3M_BATCHED split");
         for (int q = 0; q < 3; ++q) {
             ta3.push_back('N');
             tb3.push_back('N');
-            m3.push_back(m[p]);
-            n3.push_back(n[p]);
-            k3.push_back(k[p]);
-            la3.push_back(m[p] > 1 ? m[p] : 1);
-            lb3.push_back(k[p] > 1 ? k[p] : 1);
-            lc3.push_back(m[p] > 1 ? m[p] : 1);
+            m3.push_back(mp);
+            n3.push_back(np);
+            k3.push_back(kp);
+            la3.push_back(mp > 1 ? mp : 1);
+            lb3.push_back(kp > 1 ? kp : 1);
+            lc3.push_back(mp > 1 ? mp : 1);
             al3.push_back(1.0);
             be3.push_back(0.0);
-            A3.push_back(pa + q * (size_t)m[p] * k[p]);
-            B3.push_back(pb + q * (size_t)k[p] * n[p]);
-            C3.push_back(pt + q * (size_t)m[p] * n[p]);
+            A3.push_back(pa + q * (size_t)mp * kp);
+            B3.push_back(pb + q * (size_t)kp * np);
+            C3.push_back(pt + q * (size_t)mp * np);
         }
     }
     void *plan = grouped_plan_create(GROUPED_F64, 3 * count, ta3.data(), tb3.data(), m3.data(), n3.data(), k3.data(),
@@ -125,8 +134,9 @@ int gemm_amd_zgemm3m_batched(int count, const char *transa, const char *transb, 
     if (!plan) return hip_status((hipError_t)st, "ZGEMM3M_BATCHED plan");
     int e = grouped_plan_launch(plan, s);
     for (int p = 0; p < count && !e; ++p) {
-        const double *pt = ws + off[p] + 3 * ((size_t)m[p] * k[p] + (size_t)k[p] * n[p]);
-        e = launch_gemm3m_combine_d(m[p], n[p], pt, alpha + 2 * p, beta + 2 * p, c[p], ldc[p], s);
+        const int mp = even(m[p]), np = even(n[p]), kp = even(k[p]);
+        const double *pt = ws + off[p] + 3 * ((size_t)mp * kp + (size_t)kp * np);
+        e = launch_gemm3m_combine_d(m[p], n[p], mp, np, pt, alpha + 2 * p, beta + 2 * p, c[p], ldc[p], s);
     }
     hipError_t se = hipStreamSynchronize(s);   // the plan's table must outlive the launch
     grouped_plan_destroy(plan);

@@ -17,31 +17,37 @@
 
 namespace gemm_amd {
 
-// planes[0..2] of op(X) (rows x cols, ld rows): re, im, re+im.  op = N, T or
-// C (conjugate transpose).  X is interleaved complex with leading dim ldx.
+// planes[0..2] of op(X) (rows x cols): re, im, re+im, each prows x pcols
+// (ld prows) with zeros past rows and cols.  op = N, T or C (conjugate
+// transpose).  X is interleaved complex with leading dim ldx.
 template <typename T>
-__global__ __launch_bounds__(256) void split_planes_kernel(int rows, int cols, const T *__restrict__ X, int64_t ldx,
-                                                           bool trans, bool conj, T *__restrict__ planes) {
-    const int64_t total = (int64_t)rows * cols;
+__global__ __launch_bounds__(256) void split_planes_kernel(int rows, int cols, int prows, int pcols,
+                                                           const T *__restrict__ X, int64_t ldx, bool trans, bool conj,
+                                                           T *__restrict__ planes) {
+    const int64_t total = (int64_t)prows * pcols;
     for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
-        const int64_t r = idx % rows, c = idx / rows;
-        const int64_t src = trans ? (c + r * ldx) : (r + c * ldx);
-        const T re = X[2 * src];
-        T im = X[2 * src + 1];
-        if (conj) im = -im;
+        const int64_t r = idx % prows, c = idx / prows;
+        T re = T(0), im = T(0);
+        if (r < rows && c < cols) {
+            const int64_t src = trans ? (c + r * ldx) : (r + c * ldx);
+            re = X[2 * src];
+            im = X[2 * src + 1];
+            if (conj) im = -im;
+        }
         planes[idx] = re;
         planes[idx + total] = im;
         planes[idx + 2 * total] = re + im;
     }
 }
 
+// t: the three real products, each mt x nt (ld mt, mt >= m, nt >= n).
 template <typename T>
-__global__ __launch_bounds__(256) void combine3m_kernel(int m, int n, const T *__restrict__ t, T ar, T ai, T br, T bi,
-                                                        T *__restrict__ C, int64_t ldc) {
-    const int64_t total = (int64_t)m * n;
+__global__ __launch_bounds__(256) void combine3m_kernel(int m, int n, int mt, int nt, const T *__restrict__ t, T ar,
+                                                        T ai, T br, T bi, T *__restrict__ C, int64_t ldc) {
+    const int64_t total = (int64_t)m * n, plane = (int64_t)mt * nt;
     for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
-        const int64_t i = idx % m, j = idx / m;
-        const T t1 = t[idx], t2 = t[idx + total], t3 = t[idx + 2 * total];
+        const int64_t i = idx % m, j = idx / m, ti = i + j * mt;
+        const T t1 = t[ti], t2 = t[ti + plane], t3 = t[ti + 2 * plane];
         const T re = t1 - t2, im = t3 - t1 - t2;
         T *c = C + 2 * (i + j * ldc);
         T cr = ar * re - ai * im, ci = ar * im + ai * re;
@@ -97,10 +103,10 @@ static int gemm3m_t(char ta, char tb, int m, int n, int k, const T *alpha, const
     T *pt = pb + 3 * (size_t)k * n;
     const bool tA = !(ta == 'N' || ta == 'n'), cA = (ta == 'C' || ta == 'c');
     const bool tB = !(tb == 'N' || tb == 'n'), cB = (tb == 'C' || tb == 'c');
-    hipLaunchKernelGGL(split_planes_kernel<T>, dim3(grid_of((int64_t)m * k)), dim3(256), 0, s, m, k, A, lda, tA, cA,
-                       pa);
-    hipLaunchKernelGGL(split_planes_kernel<T>, dim3(grid_of((int64_t)k * n)), dim3(256), 0, s, k, n, B, ldb, tB, cB,
-                       pb);
+    hipLaunchKernelGGL(split_planes_kernel<T>, dim3(grid_of((int64_t)m * k)), dim3(256), 0, s, m, k, m, k, A, lda, tA,
+                       cA, pa);
+    hipLaunchKernelGGL(split_planes_kernel<T>, dim3(grid_of((int64_t)k * n)), dim3(256), 0, s, k, n, k, n, B, ldb, tB,
+                       cB, pb);
     if (hipError_t e = hipGetLastError()) return (int)e;
     const size_t mk = (size_t)m * k, kn = (size_t)k * n, mn = (size_t)m * n;
     for (int p = 0; p < 3; ++p) {
@@ -113,8 +119,8 @@ static int gemm3m_t(char ta, char tb, int m, int n, int k, const T *alpha, const
                              gemm_ws, s);
         if (e) return e;
     }
-    hipLaunchKernelGGL(combine3m_kernel<T>, dim3(grid_of((int64_t)m * n)), dim3(256), 0, s, m, n, pt, ar, ai, br, bi,
-                       C, ldc);
+    hipLaunchKernelGGL(combine3m_kernel<T>, dim3(grid_of((int64_t)m * n)), dim3(256), 0, s, m, n, m, n, pt, ar, ai, br,
+                       bi, C, ldc);
     return (int)hipGetLastError();
 }
 
@@ -131,26 +137,27 @@ int launch_cgemm3m(char ta, char tb, int m, int n, int k, const float *alpha, co
 }
 
 // Pieces of the 3M method for callers that batch the three real products
-// themselves (the grouped zgemm3m batch): planes of op(A) (3 x m*k) and op(B)
-// (3 x k*n), and the final combine of the products t (3 x m*n).
-int launch_gemm3m_split_d(char ta, char tb, int m, int n, int k, const double *A, int64_t lda, const double *B,
-                          int64_t ldb, double *pa, double *pb, hipStream_t s) {
+// themselves (the grouped zgemm3m batch): zero-padded planes of op(A)
+// (3 x mp*kp) and op(B) (3 x kp*np), and the final combine of the products t
+// (3 x mp*np).
+int launch_gemm3m_split_d(char ta, char tb, int m, int n, int k, int mp, int np, int kp, const double *A,
+                          int64_t lda, const double *B, int64_t ldb, double *pa, double *pb, hipStream_t s) {
     const bool tA = !(ta == 'N' || ta == 'n'), cA = (ta == 'C' || ta == 'c');
     const bool tB = !(tb == 'N' || tb == 'n'), cB = (tb == 'C' || tb == 'c');
-    if ((int64_t)m * k > 0)
-        hipLaunchKernelGGL(split_planes_kernel<double>, dim3(grid_of((int64_t)m * k)), dim3(256), 0, s, m, k, A, lda,
-                           tA, cA, pa);
-    if ((int64_t)k * n > 0)
-        hipLaunchKernelGGL(split_planes_kernel<double>, dim3(grid_of((int64_t)k * n)), dim3(256), 0, s, k, n, B, ldb,
-                           tB, cB, pb);
+    if ((int64_t)mp * kp > 0)
+        hipLaunchKernelGGL(split_planes_kernel<double>, dim3(grid_of((int64_t)mp * kp)), dim3(256), 0, s, m, k, mp, kp,
+                           A, lda, tA, cA, pa);
+    if ((int64_t)kp * np > 0)
+        hipLaunchKernelGGL(split_planes_kernel<double>, dim3(grid_of((int64_t)kp * np)), dim3(256), 0, s, k, n, kp, np,
+                           B, ldb, tB, cB, pb);
     return (int)hipGetLastError();
 }
 
-int launch_gemm3m_combine_d(int m, int n, const double *t, const double *alpha, const double *beta, double *C,
-                            int64_t ldc, hipStream_t s) {
+int launch_gemm3m_combine_d(int m, int n, int mp, int np, const double *t, const double *alpha, const double *beta,
+                            double *C, int64_t ldc, hipStream_t s) {
     if ((int64_t)m * n == 0) return 0;
-    hipLaunchKernelGGL(combine3m_kernel<double>, dim3(grid_of((int64_t)m * n)), dim3(256), 0, s, m, n, t, alpha[0],
-                       alpha[1], beta[0], beta[1], C, ldc);
+    hipLaunchKernelGGL(combine3m_kernel<double>, dim3(grid_of((int64_t)m * n)), dim3(256), 0, s, m, n, mp, np, t,
+                       alpha[0], alpha[1], beta[0], beta[1], C, ldc);
     return (int)hipGetLastError();
 }
 

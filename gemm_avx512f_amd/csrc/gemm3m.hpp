@@ -20,8 +20,9 @@ int launch_cgemm3m(char ta, char tb, int m, int n, int k, const float *alpha, co
 }  // namespace gemm_amd
 
 namespace gemm_amd {
-int launch_gemm3m_split_d(char ta, char tb, int m, int n, int k, const double *A, int64_t lda, const double *B,
-                          int64_t ldb, double *pa, double *pb, hipStream_t s);
-int launch_gemm3m_combine_d(int m, int n, const double *t, const double *alpha, const double *beta, double *C,
-                            int64_t ldc, hipStream_t s);
+// planes padded to mp x kp (A) and kp x np (B) with zeros; t is 3 x mp*np
+int launch_gemm3m_split_d(char ta, char tb, int m, int n, int k, int mp, int np, int kp, const double *A,
+                          int64_t lda, const double *B, int64_t ldb, double *pa, double *pb, hipStream_t s);
+int launch_gemm3m_combine_d(int m, int n, int mp, int np, const double *t, const double *alpha, const double *beta,
+                            double *C, int64_t ldc, hipStream_t s);
 }  // namespace gemm_amd
