@@ -37,21 +37,22 @@ struct GemmParams {
 // instruction count of a k-pair gather (measured 57.1 vs 53.7 TFLOP/s at
 // 8192^3 NN).
 //
-// KMAJOR (FP64 rows-contiguous operands only): the slab is stored [k][row]
-// with a row pitch of KPITCH = ROWS + 4 doubles instead, so the 16-byte
-// register of two neighbouring rows lands as one conflict-free LDS store
-// (the transposing store of the [row][k] layout puts 8 lanes on each bank
-// pair).
+// KMAJOR (rows-contiguous FP64 and bf16 operands): the slab is stored
+// [k][row] with a row pitch of KPITCH = ROWS + 4 doubles (ROWS + 8 bf16)
+// instead, so the 16-byte register of neighbouring rows lands as one
+// conflict-free LDS store (the transposing store of the [row][k] layout puts
+// 8 lanes on each FP64 bank pair, and writes bf16 two bytes at a time).  The
+// bf16 wave reads such a slab with LDS transpose reads (WaveBF16).
 template <typename T, int ROWS, int BK, int NT, bool KCONTIG, bool VEC, bool KMAJOR = false>
 struct TileLoader {
     using V = typename Vec16<T>::t;
     static constexpr int VW = Vec16<T>::N;
     static constexpr int PITCH = BK + VW;
-    static constexpr int KPITCH = ROWS + 4;
+    static constexpr int KPITCH = ROWS + (sizeof(T) == 2 ? 8 : 4);
     static constexpr int SIZE = KMAJOR ? BK * KPITCH : ROWS * PITCH;   // elements per stage
     static constexpr int NVEC = ROWS * BK / VW;
     static constexpr bool KGATHER = !KCONTIG && sizeof(T) == 4;
-    static_assert(!KMAJOR || (!KCONTIG && sizeof(T) == 8), "k-major slabs are for rows-contiguous FP64 operands");
+    static_assert(!KMAJOR || (!KCONTIG && sizeof(T) != 4), "k-major slabs are for rows-contiguous FP64 / bf16 operands");
     static_assert(NVEC % NT == 0, "tile must split evenly over the workgroup");
     static constexpr int PER = NVEC / NT;
     V reg[PER];
@@ -371,20 +372,55 @@ struct WaveF32 {
 // BF16 inputs, FP32 accumulation: v_mfma_f32_32x32x16_bf16.  Operands are bf16
 // bit patterns (uint16_t) in LDS rows of BK + 8 elements; lane group
 // g = lane/32 supplies k = 8g..8g+7 of every 16-deep MFMA step.
-template <int WTM, int WTN, int BK>
+//
+// AKP / BKP > 0: that operand's slab is k-major ([k][row], row pitch AKP / BKP
+// elements) and is read with ds_read_b64_tr_b16.  That instruction works on
+// groups of 16 lanes: lane i of a group receives, as its element e (0..3),
+// element i % 4 of the 8-byte word addressed by lane 4e + i / 4 of the group
+// (checked on gfx950 with an index-valued LDS probe).  So if lane j of the
+// group addresses (k0 + j / 4, r0 + 4 (j % 4)), lane i receives rows r0 + i at
+// k0 .. k0+3: two such reads (k0 and k0 + 4) give a lane its 8 consecutive k.
+template <int WTM, int WTN, int BK, int AKP = 0, int BKP = 0>
 struct WaveBF16 {
     static constexpr int SM = WTM / 32, SN = WTN / 32;
     static constexpr int PITCH = BK + 8;
     static_assert(BK % 16 == 0, "bf16 slab depth must be a multiple of 16");
+    static_assert(AKP % 4 == 0 && BKP % 4 == 0, "transpose reads need 8-byte aligned rows");
     f16_t acc[SM][SN];
     int aoff[SM], boff[SN];
 
+    // offset of the lane's fragment in a 32-row block starting at row `row0`
+    template <int KP>
+    __device__ __forceinline__ static int frag_off(int lane, int row0) {
+        if constexpr (KP > 0) {
+            const int i = lane & 15, grp = lane >> 4;
+            return (8 * (grp >> 1) + (i >> 2)) * KP + row0 + 16 * (grp & 1) + 4 * (i & 3);
+        } else {
+            return (row0 + (lane & 31)) * PITCH + 8 * (lane >> 5);
+        }
+    }
+    // the lane's 8 k values of 16-deep k-step s
+    template <int KP>
+    __device__ __forceinline__ static bf8_t frag(const uint16_t *S, int off, int s) {
+        if constexpr (KP > 0) {
+            typedef short s4_t __attribute__((ext_vector_type(4)));
+            typedef short s8_t __attribute__((ext_vector_type(8)));
+            using LP = __attribute__((address_space(3))) s4_t *;
+            const uint16_t *p = S + off + 16 * s * KP;
+            const s4_t lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((LP)(p));
+            const s4_t hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((LP)(p + 4 * KP));
+            const s8_t v = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+            return __builtin_bit_cast(bf8_t, v);
+        } else {
+            return *reinterpret_cast<const bf8_t *>(S + off + 16 * s);
+        }
+    }
+
     __device__ __forceinline__ void init(int lane, int wm0, int wn0) {
-        const int q = lane & 31, g = lane >> 5;
 #pragma unroll
-        for (int sm = 0; sm < SM; ++sm) aoff[sm] = (wm0 + sm * 32 + q) * PITCH + 8 * g;
+        for (int sm = 0; sm < SM; ++sm) aoff[sm] = frag_off<AKP>(lane, wm0 + sm * 32);
 #pragma unroll
-        for (int sn = 0; sn < SN; ++sn) boff[sn] = (wn0 + sn * 32 + q) * PITCH + 8 * g;
+        for (int sn = 0; sn < SN; ++sn) boff[sn] = frag_off<BKP>(lane, wn0 + sn * 32);
 #pragma unroll
         for (int sm = 0; sm < SM; ++sm)
 #pragma unroll
@@ -398,9 +434,9 @@ struct WaveBF16 {
         for (int s = 0; s < BK / 16; ++s) {
             bf8_t av[SM], bv[SN];
 #pragma unroll
-            for (int sm = 0; sm < SM; ++sm) av[sm] = *reinterpret_cast<const bf8_t *>(As + aoff[sm] + 16 * s);
+            for (int sm = 0; sm < SM; ++sm) av[sm] = frag<AKP>(As, aoff[sm], s);
 #pragma unroll
-            for (int sn = 0; sn < SN; ++sn) bv[sn] = *reinterpret_cast<const bf8_t *>(Bs + boff[sn] + 16 * s);
+            for (int sn = 0; sn < SN; ++sn) bv[sn] = frag<BKP>(Bs, boff[sn], s);
 #pragma unroll
             for (int sm = 0; sm < SM; ++sm)
 #pragma unroll

@@ -43,13 +43,18 @@ __device__ __forceinline__ void grouped_tile(const GroupDesc &d, int tm, int tn,
                                              int tid) {   // d: in global memory
     constexpr int NT = WM * WN * 64;
     constexpr int WTM = BM / WM, WTN = BN / WN;
-    // vector loads where the problem's operands allow them (a run-time choice:
-    // eight compile-time variants of the tile would crowd the register file)
-    using LA = TileLoader<TI, BM, BK, NT, TA, true>;
-    using LB = TileLoader<TI, BN, BK, NT, !TB, true>;
-    using Wave = typename WaveSel<TI, WTM, WTN, BK>::type;
+    // bf16 rows-contiguous operands are staged k-major (16-byte stores of 8
+    // rows) and read with LDS transpose reads; FP32 and FP64 keep the
+    // single-problem kernel's 64x64 staging
+    constexpr bool BF = sizeof(TI) == 2;
+    constexpr bool KMA = BF && !TA, KMB = BF && TB;
+    using LA = TileLoader<TI, BM, BK, NT, TA, true, KMA>;
+    using LB = TileLoader<TI, BN, BK, NT, !TB, true, KMB>;
+    using Wave = std::conditional_t<BF, WaveBF16<WTM, WTN, BK, KMA ? LA::KPITCH : 0, KMB ? LB::KPITCH : 0>,
+                                    typename WaveSel<TI, WTM, WTN, BK>::type>;
     constexpr int PITCH = LA::PITCH;
     constexpr int STAGE = (BM + BN) * PITCH;
+    static_assert(LA::SIZE == BM * PITCH && LB::SIZE == BN * PITCH, "k-major slabs must fill the same stage");
     TI *smem = reinterpret_cast<TI *>(smem_raw);
     const TI *A = static_cast<const TI *>(d.A);
     const TI *B = static_cast<const TI *>(d.B);
