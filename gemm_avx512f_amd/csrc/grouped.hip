@@ -43,18 +43,21 @@ __device__ __forceinline__ void grouped_tile(const GroupDesc &d, int tm, int tn,
                                              int tid) {   // d: in global memory
     constexpr int NT = WM * WN * 64;
     constexpr int WTM = BM / WM, WTN = BN / WN;
-    // bf16 rows-contiguous operands are staged k-major (16-byte stores of 8
-    // rows) and read with LDS transpose reads; FP32 and FP64 keep the
-    // single-problem kernel's 64x64 staging
-    constexpr bool BF = sizeof(TI) == 2;
-    constexpr bool KMA = BF && !TA, KMB = BF && TB;
+    // rows-contiguous FP64 and bf16 operands are staged k-major (one 16-byte
+    // store per register); FP64 reads them as the single-problem kernel does
+    // (row / column pairs), bf16 with LDS transpose reads.  FP32 gathers them
+    // k-wise.  The per-element arithmetic is the single-problem kernel's.
+    constexpr bool BF = sizeof(TI) == 2, DP = sizeof(TI) == 8;
+    constexpr bool KMA = (BF || DP) && !TA, KMB = (BF || DP) && TB;
     using LA = TileLoader<TI, BM, BK, NT, TA, true, KMA>;
     using LB = TileLoader<TI, BN, BK, NT, !TB, true, KMB>;
-    using Wave = std::conditional_t<BF, WaveBF16<WTM, WTN, BK, KMA ? LA::KPITCH : 0, KMB ? LB::KPITCH : 0>,
-                                    typename WaveSel<TI, WTM, WTN, BK>::type>;
-    constexpr int PITCH = LA::PITCH;
-    constexpr int STAGE = (BM + BN) * PITCH;
-    static_assert(LA::SIZE == BM * PITCH && LB::SIZE == BN * PITCH, "k-major slabs must fill the same stage");
+    constexpr int AKP = KMA ? LA::KPITCH : 0, BKP = KMB ? LB::KPITCH : 0;
+    using Wave = std::conditional_t<
+        BF, WaveBF16<WTM, WTN, BK, AKP, BKP>,
+        std::conditional_t<DP, WaveF64<WTM, WTN, BK, AKP, BKP>, typename WaveSel<TI, WTM, WTN, BK>::type>>;
+    constexpr int BOFF = LA::SIZE;               // B's slab follows A's in a stage
+    constexpr int STAGE = LA::SIZE + LB::SIZE;   // at most (BM + BN) * (BK + 16 bytes), what the launch reserves
+    static_assert(STAGE <= (BM + BN) * LA::PITCH, "stage exceeds the reserved LDS");
     TI *smem = reinterpret_cast<TI *>(smem_raw);
     const TI *A = static_cast<const TI *>(d.A);
     const TI *B = static_cast<const TI *>(d.B);
@@ -94,18 +97,18 @@ __device__ __forceinline__ void grouped_tile(const GroupDesc &d, int tm, int tn,
                 TI *nxt = smem + ((kt + 1) & 1) * STAGE;
                 fetch(ia, ib, kt + 2);
                 __builtin_amdgcn_sched_barrier(0);   // issue the loads before the slab's work
-                w.compute(cur, cur + BM * PITCH);
+                w.compute(cur, cur + BOFF);
                 // unconditional (after the last slab it fills the idle stage):
                 // a skipped store would leave this set's loads pending on one
                 // path and cost exact wait counts everywhere after it
                 sa.store(nxt, tid);
-                sb.store(nxt + BM * PITCH, tid);
+                sb.store(nxt + BOFF, tid);
                 lds_barrier();   // slab kt+2 stays in flight
             };
             fetch(la0, lb0, 0);
             fetch(la1, lb1, 1);
             la0.store(smem, tid);
-            lb0.store(smem + BM * PITCH, tid);
+            lb0.store(smem + BOFF, tid);
             lds_barrier();
             // one exit, at the loop test: every path round the loop runs both
             // steps and carries exactly one set in flight (la1/lb1), so the
@@ -126,9 +129,9 @@ __device__ __forceinline__ void grouped_tile(const GroupDesc &d, int tm, int tn,
             la0.load_buf(ra, aoff + nf * astep, d.lda, mlim, krem, tid);
             lb0.load_buf(rb, boff + nf * bstep, d.ldb, nlim, krem, tid);
             la0.store(st, tid);
-            lb0.store(st + BM * PITCH, tid);
+            lb0.store(st + BOFF, tid);
             lds_barrier();
-            w.compute(st, st + BM * PITCH);
+            w.compute(st, st + BOFF);
         }
     } else {
         // range-checked element path, one slab ahead (its branches would make a
@@ -140,7 +143,7 @@ __device__ __forceinline__ void grouped_tile(const GroupDesc &d, int tm, int tn,
             la.load(abase, d.lda, mlim, d.k, tid);
             lb.load(bbase, d.ldb, nlim, d.k, tid);
             la.store(smem, tid);
-            lb.store(smem + BM * PITCH, tid);
+            lb.store(smem + BOFF, tid);
         }
         __syncthreads();
         for (int kt = 0; kt < nk; ++kt) {
@@ -152,10 +155,10 @@ __device__ __forceinline__ void grouped_tile(const GroupDesc &d, int tm, int tn,
                 la.load(abase + (kt + 1) * astep, d.lda, mlim, krem, tid);
                 lb.load(bbase + (kt + 1) * bstep, d.ldb, nlim, krem, tid);
             }
-            w.compute(cur, cur + BM * PITCH);
+            w.compute(cur, cur + BOFF);
             if (more) {
                 la.store(nxt, tid);
-                lb.store(nxt + BM * PITCH, tid);
+                lb.store(nxt + BOFF, tid);
             }
             __syncthreads();
         }
