@@ -96,9 +96,17 @@ int gemm_amd_zgemm3m_batched(int count, const char *transa, const char *transb, 
         const size_t mp = even(m[p]), np = even(n[p]), kp = even(k[p]);
         off[p + 1] = off[p] + 3 * (mp * kp + kp * np + mp * np);
     }
+    // then the split and combine job tables
+    const size_t tab = (off[count] * sizeof(double) + 255) / 256 * 256;
+    const size_t tabs = 2 * (size_t)count * sizeof(SplitJob3m), tabc = (size_t)count * sizeof(CombineJob3m);
     int st = 0;
-    double *ws = (double *)device_workspace(WS_COMPLEX, off[count] * sizeof(double) + 256, &st);
+    char *wsb = (char *)device_workspace(WS_COMPLEX, tab + tabs + tabc + 256, &st);
     if (st) return st;
+    double *ws = (double *)wsb;
+    SplitJob3m *d_split = (SplitJob3m *)(wsb + tab);
+    CombineJob3m *d_comb = (CombineJob3m *)(wsb + tab + tabs);
+    std::vector<SplitJob3m> sj(2 * (size_t)count);
+    std::vector<CombineJob3m> cj(count);
     std::vector<char> ta3, tb3;
     std::vector<int> m3, n3, k3, la3, lb3, lc3;
     std::vector<double> al3, be3;
@@ -109,9 +117,11 @@ int gemm_amd_zgemm3m_batched(int count, const char *transa, const char *transb, 
         double *pa = ws + off[p];
         double *pb = pa + 3 * (size_t)mp * kp;
         double *pt = pb + 3 * (size_t)kp * np;
-        int e = launch_gemm3m_split_d(transa[p], transb[p], m[p], n[p], k[p], mp, np, kp, a[p], lda[p], b[p], ldb[p],
-                                      pa, pb, s);
-        if (e) return hip_status((hipError_t)e, "ZGEMM3M_BATCHED split");
+        const bool tA = !(transa[p] == 'N' || transa[p] == 'n'), cA = (transa[p] == 'C' || transa[p] == 'c');
+        const bool tB = !(transb[p] == 'N' || transb[p] == 'n'), cB = (transb[p] == 'C' || transb[p] == 'c');
+        sj[2 * p] = {m[p], k[p], mp, kp, tA, cA, a[p], lda[p], pa};
+        sj[2 * p + 1] = {k[p], n[p], kp, np, tB, cB, b[p], ldb[p], pb};
+        cj[p] = {m[p], n[p], mp, np, pt, alpha[2 * p], alpha[2 * p + 1], beta[2 * p], beta[2 * p + 1], c[p], ldc[p]};
         for (int q = 0; q < 3; ++q) {
             ta3.push_back('N');
             tb3.push_back('N');
@@ -132,13 +142,13 @@ int gemm_amd_zgemm3m_batched(int count, const char *transa, const char *transb, 
                                      al3.data(), A3.data(), la3.data(), B3.data(), lb3.data(), be3.data(), C3.data(),
                                      lc3.data(), rt().cus, &st);
     if (!plan) return hip_status((hipError_t)st, "ZGEMM3M_BATCHED plan");
-    int e = grouped_plan_launch(plan, s);
-    for (int p = 0; p < count && !e; ++p) {
-        const int mp = even(m[p]), np = even(n[p]), kp = even(k[p]);
-        const double *pt = ws + off[p] + 3 * ((size_t)mp * kp + (size_t)kp * np);
-        e = launch_gemm3m_combine_d(m[p], n[p], mp, np, pt, alpha + 2 * p, beta + 2 * p, c[p], ldc[p], s);
-    }
-    hipError_t se = hipStreamSynchronize(s);   // the plan's table must outlive the launch
+    // the host tables stay alive until the synchronisation below
+    int e = (int)hipMemcpyAsync(d_split, sj.data(), tabs, hipMemcpyHostToDevice, s);
+    if (!e) e = (int)hipMemcpyAsync(d_comb, cj.data(), tabc, hipMemcpyHostToDevice, s);
+    if (!e) e = launch_gemm3m_split_batch_d(d_split, 2 * count, s);
+    if (!e) e = grouped_plan_launch(plan, s);
+    if (!e) e = launch_gemm3m_combine_batch_d(d_comb, count, s);
+    hipError_t se = hipStreamSynchronize(s);   // the plan's and the jobs' tables must outlive the launches
     grouped_plan_destroy(plan);
     if (e) return hip_status((hipError_t)e, "ZGEMM3M_BATCHED");
     return hip_status(se, "ZGEMM3M_BATCHED");

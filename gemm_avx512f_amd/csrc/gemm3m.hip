@@ -20,10 +20,12 @@ namespace gemm_amd {
 // planes[0..2] of op(X) (rows x cols): re, im, re+im, each prows x pcols
 // (ld prows) with zeros past rows and cols.  op = N, T or C (conjugate
 // transpose).  X is interleaved complex with leading dim ldx.
-template <typename T>
-__global__ __launch_bounds__(256) void split_planes_kernel(int rows, int cols, int prows, int pcols,
-                                                           const T *__restrict__ X, int64_t ldx, bool trans, bool conj,
-                                                           T *__restrict__ planes) {
+// The element loops are shared by the one-problem kernels and the batched
+// ones (one launch over a job table, blockIdx.y = job); XP/PP/TP/CP are the
+// pointer types (global-address-space ones for pointers read from a table).
+template <typename T, typename XP, typename PP>
+__device__ __forceinline__ void split_range(int rows, int cols, int prows, int pcols, XP X, int64_t ldx, bool trans,
+                                            bool conj, PP planes) {
     const int64_t total = (int64_t)prows * pcols;
     for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
         const int64_t r = idx % prows, c = idx / prows;
@@ -40,16 +42,23 @@ __global__ __launch_bounds__(256) void split_planes_kernel(int rows, int cols, i
     }
 }
 
-// t: the three real products, each mt x nt (ld mt, mt >= m, nt >= n).
 template <typename T>
-__global__ __launch_bounds__(256) void combine3m_kernel(int m, int n, int mt, int nt, const T *__restrict__ t, T ar,
-                                                        T ai, T br, T bi, T *__restrict__ C, int64_t ldc) {
+__global__ __launch_bounds__(256) void split_planes_kernel(int rows, int cols, int prows, int pcols,
+                                                           const T *__restrict__ X, int64_t ldx, bool trans, bool conj,
+                                                           T *__restrict__ planes) {
+    split_range<T>(rows, cols, prows, pcols, X, ldx, trans, conj, planes);
+}
+
+// t: the three real products, each mt x nt (ld mt, mt >= m, nt >= n).
+template <typename T, typename TP, typename CP>
+__device__ __forceinline__ void combine_range(int m, int n, int mt, int nt, TP t, T ar, T ai, T br, T bi, CP C,
+                                              int64_t ldc) {
     const int64_t total = (int64_t)m * n, plane = (int64_t)mt * nt;
     for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
         const int64_t i = idx % m, j = idx / m, ti = i + j * mt;
         const T t1 = t[ti], t2 = t[ti + plane], t3 = t[ti + 2 * plane];
         const T re = t1 - t2, im = t3 - t1 - t2;
-        T *c = C + 2 * (i + j * ldc);
+        const auto c = C + 2 * (i + j * ldc);
         T cr = ar * re - ai * im, ci = ar * im + ai * re;
         if (br != T(0) || bi != T(0)) {
             const T xr = c[0], xi = c[1];
@@ -59,6 +68,23 @@ __global__ __launch_bounds__(256) void combine3m_kernel(int m, int n, int mt, in
         c[0] = cr;
         c[1] = ci;
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void combine3m_kernel(int m, int n, int mt, int nt, const T *__restrict__ t, T ar,
+                                                        T ai, T br, T bi, T *__restrict__ C, int64_t ldc) {
+    combine_range<T>(m, n, mt, nt, t, ar, ai, br, bi, C, ldc);
+}
+
+__global__ __launch_bounds__(256) void split_planes_batch_kernel(const SplitJob3m *__restrict__ jobs) {
+    const SplitJob3m &j = jobs[blockIdx.y];
+    split_range<double>(j.rows, j.cols, j.prows, j.pcols, as_global(j.X), j.ldx, j.trans != 0, j.conj != 0,
+                        as_global(j.planes));
+}
+
+__global__ __launch_bounds__(256) void combine3m_batch_kernel(const CombineJob3m *__restrict__ jobs) {
+    const CombineJob3m &j = jobs[blockIdx.y];
+    combine_range<double>(j.m, j.n, j.mt, j.nt, as_global(j.t), j.ar, j.ai, j.br, j.bi, as_global(j.C), j.ldc);
 }
 
 template <typename T>
@@ -137,27 +163,22 @@ int launch_cgemm3m(char ta, char tb, int m, int n, int k, const float *alpha, co
 }
 
 // Pieces of the 3M method for callers that batch the three real products
-// themselves (the grouped zgemm3m batch): zero-padded planes of op(A)
-// (3 x mp*kp) and op(B) (3 x kp*np), and the final combine of the products t
-// (3 x mp*np).
-int launch_gemm3m_split_d(char ta, char tb, int m, int n, int k, int mp, int np, int kp, const double *A,
-                          int64_t lda, const double *B, int64_t ldb, double *pa, double *pb, hipStream_t s) {
-    const bool tA = !(ta == 'N' || ta == 'n'), cA = (ta == 'C' || ta == 'c');
-    const bool tB = !(tb == 'N' || tb == 'n'), cB = (tb == 'C' || tb == 'c');
-    if ((int64_t)mp * kp > 0)
-        hipLaunchKernelGGL(split_planes_kernel<double>, dim3(grid_of((int64_t)mp * kp)), dim3(256), 0, s, m, k, mp, kp,
-                           A, lda, tA, cA, pa);
-    if ((int64_t)kp * np > 0)
-        hipLaunchKernelGGL(split_planes_kernel<double>, dim3(grid_of((int64_t)kp * np)), dim3(256), 0, s, k, n, kp, np,
-                           B, ldb, tB, cB, pb);
+// themselves (the grouped zgemm3m batch): one launch splits every listed
+// operand into its zero-padded planes, one launch combines every problem's
+// products (job tables in device memory; 64 workgroups per job).
+int launch_gemm3m_split_batch_d(const SplitJob3m *jobs, int njobs, hipStream_t s) {
+    for (int j0 = 0; j0 < njobs; j0 += 65535) {
+        const int nj = njobs - j0 < 65535 ? njobs - j0 : 65535;
+        hipLaunchKernelGGL(split_planes_batch_kernel, dim3(64, nj), dim3(256), 0, s, jobs + j0);
+    }
     return (int)hipGetLastError();
 }
 
-int launch_gemm3m_combine_d(int m, int n, int mp, int np, const double *t, const double *alpha, const double *beta,
-                            double *C, int64_t ldc, hipStream_t s) {
-    if ((int64_t)m * n == 0) return 0;
-    hipLaunchKernelGGL(combine3m_kernel<double>, dim3(grid_of((int64_t)m * n)), dim3(256), 0, s, m, n, mp, np, t,
-                       alpha[0], alpha[1], beta[0], beta[1], C, ldc);
+int launch_gemm3m_combine_batch_d(const CombineJob3m *jobs, int njobs, hipStream_t s) {
+    for (int j0 = 0; j0 < njobs; j0 += 65535) {
+        const int nj = njobs - j0 < 65535 ? njobs - j0 : 65535;
+        hipLaunchKernelGGL(combine3m_batch_kernel, dim3(64, nj), dim3(256), 0, s, jobs + j0);
+    }
     return (int)hipGetLastError();
 }
 

@@ -20,9 +20,25 @@ int launch_cgemm3m(char ta, char tb, int m, int n, int k, const float *alpha, co
 }  // namespace gemm_amd
 
 namespace gemm_amd {
-// planes padded to mp x kp (A) and kp x np (B) with zeros; t is 3 x mp*np
-int launch_gemm3m_split_d(char ta, char tb, int m, int n, int k, int mp, int np, int kp, const double *A,
-                          int64_t lda, const double *B, int64_t ldb, double *pa, double *pb, hipStream_t s);
-int launch_gemm3m_combine_d(int m, int n, int mp, int np, const double *t, const double *alpha, const double *beta,
-                            double *C, int64_t ldc, hipStream_t s);
+// One operand's split: op(X) (rows x cols, op = transpose if trans, conjugate
+// if conj) into planes re, im, re+im, each prows x pcols with zeros past
+// rows and cols.
+struct SplitJob3m {
+    int rows, cols, prows, pcols, trans, conj;
+    const double *X;
+    int64_t ldx;
+    double *planes;
+};
+// One problem's combine: C <- alpha * (T1 - T2 + i (T3 - T1 - T2)) + beta * C
+// from the products t (3 planes of mt x nt, ld mt).
+struct CombineJob3m {
+    int m, n, mt, nt;
+    const double *t;
+    double ar, ai, br, bi;
+    double *C;
+    int64_t ldc;
+};
+// job tables in device memory
+int launch_gemm3m_split_batch_d(const SplitJob3m *jobs, int njobs, hipStream_t s);
+int launch_gemm3m_combine_batch_d(const CombineJob3m *jobs, int njobs, hipStream_t s);
 }  // namespace gemm_amd
