@@ -219,7 +219,48 @@ def configs_leg(steps=20, warmup=3, seed=0):
     out["dtrsm_llnn_4096x512"] = {"tflops": round(float(m) * m * nr * steps / (ms / 1e3) / 1e12, 2),
                                   "ms_per_call": round(ms / steps, 4), "residual": resid,
                                   "flops": "M^2 N"}
+    out["dgetrf_4096"] = getrf_leg(4096, seed)
     return out
+
+
+def getrf_leg(n, seed, calls=5):
+    """LU with partial pivoting of an n x n uniform[-1,1) matrix (dgetrf
+    contract) on device pointers.  Each call factors a fresh copy of the matrix
+    (the copy is outside the HIP-event interval); work (2/3) n^3 flop; the
+    backward error ||P A - L U||_F / ||A||_F is checked on the last call."""
+    import ctypes
+    from gemm_avx512f_amd import _lib, blas
+    from gemm_avx512f_amd.device import DeviceBuffer, DeviceMatrix, Event, Stream, fill_uniform
+
+    stream = Stream()
+    A0, A = DeviceMatrix(n, n, np.float64), DeviceMatrix(n, n, np.float64)
+    fill_uniform(A0, seed + 21, 0, stream=stream)
+    stream.synchronize()
+    piv, info = DeviceBuffer(4 * n), DeviceBuffer(4)
+    nbytes = 8 * n * n
+    total = 0.0
+    for i in range(calls + 1):
+        _lib.check(_lib.lib().gemm_amd_memcpy(ctypes.c_void_p(A.ptr), ctypes.c_void_p(A0.ptr), nbytes, 3), "copy")
+        e0, e1 = Event(), Event()
+        e0.record(stream)
+        blas.dgetrf(n, n, A, piv, info, stream=stream)
+        e1.record(stream)
+        ms = e0.elapsed_ms(e1)
+        if i:   # the first call warms up
+            total += ms
+    a0, lu = A0.to_host(), A.to_host()
+    ipiv, inf = np.zeros(n, np.int32), np.zeros(1, np.int32)
+    piv.download(ipiv)
+    info.download(inf)
+    L, U = np.tril(lu, -1) + np.eye(n), np.triu(lu)
+    pa = a0.copy()
+    for j, r in enumerate(ipiv):
+        if r - 1 != j:
+            pa[[j, r - 1]] = pa[[r - 1, j]]
+    berr = float(np.linalg.norm(L @ U - pa) / np.linalg.norm(a0))
+    ms = total / calls
+    return {"tflops": round(2.0 / 3.0 * n ** 3 / (ms / 1e3) / 1e12, 2), "ms_per_call": round(ms, 3),
+            "backward_error": berr, "info": int(inf[0]), "flops": "(2/3) n^3"}
 
 
 def c4_shapes(seed, count=512):

@@ -32,6 +32,7 @@ SIGNATURES = {
     "sgemm_": (None, [c_vp] * 13),
     "dgemm_": (None, [c_vp] * 13),
     "dtrsm_": (None, [c_vp] * 11),
+    "dgetrf_": (None, [c_vp] * 6),
     "cgemm3m_": (None, [c_vp] * 13),
     "zgemm3m_": (None, [c_vp] * 13),
     "gemm_amd_cgemm3m": (c_int, [c_char, c_char, c_int, c_int, c_int, c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_vp,
@@ -42,6 +43,7 @@ SIGNATURES = {
     "gemm_amd_dtrsm": (c_int, [c_char, c_char, c_char, c_char, c_int, c_int, c_double, c_vp, c_int, c_vp, c_int,
                                c_vp]),
     "gemm_amd_dtrsm_kernel": (c_int, [c_int, c_long, c_long, c_vp, c_vp, c_vp, c_long, c_long, c_vp]),
+    "gemm_amd_dgetrf": (c_int, [c_int, c_int, c_vp, c_int, c_vp, c_vp, c_vp]),
     "gemm_amd_sgemm": (c_int, [c_char, c_char, c_int, c_int, c_int, c_float, c_vp, c_int, c_vp, c_int,
                                c_float, c_vp, c_int, c_vp]),
     "gemm_amd_dgemm": (c_int, [c_char, c_char, c_int, c_int, c_int, c_double, c_vp, c_int, c_vp, c_int,

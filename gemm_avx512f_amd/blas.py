@@ -114,6 +114,26 @@ def dtrsm_kernel(variant, M, N, sa, sb, c, LDB, LDC, stream=None):
     return _lib.check(rc, "dtrsm_kernel")
 
 
+# ------------------------------------------------------------------ LU (DGETRF)
+def dgetrf_(m, n, a, lda, ipiv):
+    """LAPACK-ABI DGETRF on host arrays: ``a`` (Fortran order, lda x n) is
+    overwritten by L and U, ``ipiv`` (int32, min(m, n)) receives the 1-based
+    pivot rows.  Returns info."""
+    info = ctypes.c_int(0)
+    if ipiv.dtype != np.int32 or not ipiv.flags["C_CONTIGUOUS"]:
+        raise TypeError("ipiv must be a contiguous int32 array")
+    _lib.lib().dgetrf_(_ref(m, ctypes.c_int), _ref(n, ctypes.c_int), _host_ptr(a, np.float64),
+                       _ref(lda, ctypes.c_int), ipiv.ctypes.data_as(ctypes.c_void_p), ctypes.byref(info))
+    return info.value
+
+
+def dgetrf(m, n, a: DeviceMatrix, ipiv_dev, info_dev, stream=None):
+    """P A = L U in place on a device matrix; ``ipiv_dev`` (min(m, n) int32) and
+    ``info_dev`` (one int32) are device buffers (anything with ``.ptr``)."""
+    rc = _lib.lib().gemm_amd_dgetrf(m, n, a.ptr, a.ld, ipiv_dev.ptr, info_dev.ptr, _stream(stream))
+    return _lib.check(rc, "dgetrf")
+
+
 # ------------------------------------------------------------------ 3M complex GEMM
 def _cpair(x, ctype):
     x = complex(x)

@@ -1,0 +1,364 @@
+// Blocked LU factorisation with partial pivoting (LAPACK dgetrf contract) on
+// device pointers: P A = L U, column-major, ipiv 1-based.
+//
+// The factorisation is recursive over columns (left half, then the U12 solve
+// and the Schur-complement GEMM, then the right half), so all but the leaf
+// columns run in the DTRSM and FP64 matrix-core GEMM kernels of this library.
+// A leaf is a panel of up to 16 columns and every row below its diagonal,
+// factored by one 1024-thread workgroup with the panel held in registers
+// (rows strided over threads).  After each leaf a chip-wide kernel applies
+// the leaf's row interchanges to every other column of the matrix, so the
+// state after every leaf is that of LAPACK's unblocked dgetf2 at the same
+// column and no deferred row-swap passes are needed.
+#include "getrf.hpp"
+#include "trsm.hpp"
+#include "tuner.hpp"
+#include <hip/hip_runtime.h>
+#include <float.h>
+
+namespace gemm_amd {
+
+namespace {
+
+constexpr int LU_THREADS = 1024;
+constexpr int LU_WAVES = LU_THREADS / 64;
+
+// Larger magnitude wins; equal magnitudes go to the smaller row (idamax takes
+// the first maximum).  A NaN magnitude never wins.
+__device__ inline bool pivot_better(double m1, int r1, double m2, int r2) {
+    return m1 > m2 || (m1 == m2 && r1 < r2);
+}
+
+// Factor columns [c0, c0 + w) of A over rows [c0, m) (T rows per thread, at
+// most W columns); getrf_swap_kernel then applies the w interchanges to the
+// other columns.  ipiv[c] receives the 1-based pivot row of column c; info the
+// first (1-based) column whose pivot is exactly zero, as dgetf2 reports it.
+template <int T, int W>
+__global__ __launch_bounds__(LU_THREADS) void getrf_leaf_kernel(double *__restrict__ A, int64_t lda, int m, int c0,
+                                                                int w, int *__restrict__ ipiv,
+                                                                int *__restrict__ info) {
+    // Two barriers per column: partials, pivot row and row cj are double
+    // buffered by column parity, so a column's reads never meet the next
+    // column's writes.
+    __shared__ double s_mag[2][LU_WAVES];
+    __shared__ int s_row[2][LU_WAVES];
+    __shared__ double s_prow[2][W];   // the pivot row's panel entries (become row c0+j)
+    __shared__ double s_crow[2][W];   // row c0+j's panel entries (move to the pivot row)
+    __shared__ int s_ipiv[W];          // pivots of the leaf, stored to ipiv after the loop (no global
+                                       // store may be outstanding at a column's barriers)
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int zero_col = -1;   // thread 0: first column of this leaf with an exactly zero pivot
+
+    double v[T][W];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int r = c0 + tid + t * LU_THREADS;
+#pragma unroll
+        for (int k = 0; k < W; ++k) v[t][k] = (r < m && k < w) ? A[r + (int64_t)(c0 + k) * lda] : 0.0;
+    }
+
+    for (int j = 0; j < w; ++j) {
+        const int cj = c0 + j, b = j & 1;
+        // 1. row cj's entries; pivot candidates over rows >= cj (first maximum of |a|)
+        double best = -1.0;
+        int brow = 0x7fffffff;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int r = c0 + tid + t * LU_THREADS;
+            if (r == cj)
+#pragma unroll
+                for (int k = 0; k < W; ++k) s_crow[b][k] = v[t][k];
+            double a = 0.0;
+#pragma unroll
+            for (int k = 0; k < W; ++k)
+                if (k == j) a = v[t][k];
+            const double mag = fabs(a);
+            if (r >= cj && r < m && pivot_better(mag, r, best, brow)) {
+                best = mag;
+                brow = r;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double om = __shfl_xor(best, off);
+            const int orow = __shfl_xor(brow, off);
+            if (pivot_better(om, orow, best, brow)) {
+                best = om;
+                brow = orow;
+            }
+        }
+        if (lane == 0) {
+            s_mag[b][wave] = best;
+            s_row[b][wave] = brow;
+        }
+        __syncthreads();
+        // 2. every wave reduces the 16 partials; the pivot row's owner publishes it
+        double bm = lane < LU_WAVES ? s_mag[b][lane] : -1.0;
+        int br = lane < LU_WAVES ? s_row[b][lane] : 0x7fffffff;
+#pragma unroll
+        for (int off = LU_WAVES / 2; off > 0; off >>= 1) {
+            const double om = __shfl_xor(bm, off);
+            const int orow = __shfl_xor(br, off);
+            if (pivot_better(om, orow, bm, br)) {
+                bm = om;
+                br = orow;
+            }
+        }
+        br = __shfl(br, 0);
+        if (br >= m) br = cj;   // no candidate compared larger (NaN column): keep the diagonal row
+        const int p = br;
+        if (tid == 0) s_ipiv[j] = p + 1;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int r = c0 + tid + t * LU_THREADS;
+            if (r == p)
+#pragma unroll
+                for (int k = 0; k < W; ++k) s_prow[b][k] = v[t][k];
+        }
+        __syncthreads();
+    // 3. swap, scale the column below the diagonal, rank-1 update of the panel
+        const double piv = s_prow[b][j];
+        if (tid == 0 && piv == 0.0 && zero_col < 0) zero_col = cj;
+        const bool recip = fabs(piv) >= DBL_MIN;
+        const double rpiv = 1.0 / piv;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int r = c0 + tid + t * LU_THREADS;
+            if (r == cj)
+#pragma unroll
+                for (int k = 0; k < W; ++k) v[t][k] = s_prow[b][k];
+            else if (r == p)
+#pragma unroll
+                for (int k = 0; k < W; ++k) v[t][k] = s_crow[b][k];
+            if (r > cj && r < m && piv != 0.0) {
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    if (k == j) v[t][k] = recip ? v[t][k] * rpiv : v[t][k] / piv;
+                }
+                double l = 0.0;
+#pragma unroll
+                for (int k = 0; k < W; ++k)
+                    if (k == j) l = v[t][k];
+#pragma unroll
+                for (int k = 0; k < W; ++k)
+                    if (k > j && k < w) v[t][k] = fma(-l, s_prow[b][k], v[t][k]);
+            }
+        }
+    }
+
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int r = c0 + tid + t * LU_THREADS;
+        if (r < m)
+#pragma unroll
+            for (int k = 0; k < W; ++k)
+                if (k < w) A[r + (int64_t)(c0 + k) * lda] = v[t][k];
+    }
+    if (tid < w) ipiv[c0 + tid] = s_ipiv[tid];
+    if (tid == 0) {
+        // leaves run in column order on one stream: the first leaf resets info
+        if (c0 == 0) *info = zero_col >= 0 ? zero_col + 1 : 0;
+        else if (zero_col >= 0 && *info == 0) *info = zero_col + 1;
+    }
+}
+
+// Applies the interchanges ipiv[c0 .. c0+w) (1-based rows, in order) to every
+// column in [0, ncols) outside [c0, c0+w).  Each workgroup first reduces the
+// w swaps to their net effect on the rows they touch (the value that ends at
+// row dst started at row src; at most 2w rows move), then 32 lanes move one
+// column: every lane of a column loads its source before any lane of that
+// column stores (one instruction each).
+constexpr int SWAP_THREADS = 256;
+
+template <int W>
+__global__ __launch_bounds__(SWAP_THREADS) void getrf_swap_kernel(double *__restrict__ A, int64_t lda, int ncols,
+                                                                  int c0, int w, const int *__restrict__ ipiv) {
+    static_assert(2 * W <= 64, "one wave holds the slots");
+    __shared__ int s_dst[2 * W], s_src[2 * W];
+    __shared__ int s_npairs;
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        // Slots 0..w-1 are rows c0..c0+w-1; pivot rows below the block get
+        // slots w.. in order of first appearance.  Lane i tracks which
+        // original row's value currently sits in slot i.
+        int prow[W], pslot[W];
+        int next = w;
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            prow[j] = j < w ? ipiv[c0 + j] - 1 : c0;
+            int sl = prow[j] - c0;
+            if (j < w && sl >= w) {
+                sl = -1;
+#pragma unroll
+                for (int q = 0; q < W; ++q)
+                    if (q < j && sl < 0 && prow[q] == prow[j]) sl = pslot[q];
+                if (sl < 0) sl = next++;
+            }
+            pslot[j] = sl;
+        }
+        const int lane = tid;
+        int row = lane < w ? c0 + lane : -1;
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+            if (j < w && pslot[j] == lane && lane >= w) row = prow[j];
+        int held = row;
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            if (j < w) {
+                const int va = __shfl(held, j), vb = __shfl(held, pslot[j]);
+                if (lane == j) held = vb;
+                if (lane == pslot[j]) held = va;
+            }
+        }
+        const bool moved = row >= 0 && held != row;
+        const unsigned long long mask = __ballot(moved);
+        if (moved) {
+            const int idx = __popcll(mask & ((1ull << lane) - 1ull));
+            s_dst[idx] = row;
+            s_src[idx] = held;
+        }
+        if (lane == 0) s_npairs = __popcll(mask);
+    }
+    __syncthreads();
+    const int np = s_npairs, pair = tid & 31;
+    const int c = blockIdx.x * (SWAP_THREADS / 32) + (tid >> 5);
+    if (pair >= np || c >= ncols || (c >= c0 && c < c0 + w)) return;
+    double *col = A + (int64_t)c * lda;
+    const double x = col[s_src[pair]];
+    col[s_dst[pair]] = x;
+}
+
+// X <- L^-1 X for a unit lower n1 x n1 triangle L (n1 <= NMAX) and an
+// n1 x n2 block X, by forward substitution: one thread per column of X, the
+// column in registers, L staged in LDS and read as broadcasts.
+template <int NMAX>
+__global__ __launch_bounds__(64) void getrf_trsm_small_kernel(const double *__restrict__ L, int64_t lda,
+                                                              double *__restrict__ X, int n1, int n2) {
+    __shared__ double s_l[NMAX * NMAX];   // column-major, leading dimension NMAX
+    const int tid = threadIdx.x;
+    for (int e = tid; e < n1 * n1; e += 64) {
+        const int i = e % n1, k = e / n1;
+        s_l[i + k * NMAX] = L[i + (int64_t)k * lda];
+    }
+    __syncthreads();
+    const int c = blockIdx.x * 64 + tid;
+    if (c >= n2) return;
+    double *col = X + (int64_t)c * lda;
+    double x[NMAX];
+#pragma unroll
+    for (int i = 0; i < NMAX; ++i) x[i] = i < n1 ? col[i] : 0.0;
+#pragma unroll
+    for (int k = 0; k < NMAX - 1; ++k) {
+#pragma unroll
+        for (int i = k + 1; i < NMAX; ++i)
+            if (i < n1) x[i] = fma(-s_l[i + k * NMAX], x[k], x[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < NMAX; ++i)
+        if (i < n1) col[i] = x[i];
+}
+
+template <int T, int W>
+int leaf(double *A, int64_t lda, int m, int ncols, int c0, int w, int *ipiv, int *info, hipStream_t s) {
+    hipLaunchKernelGGL((getrf_leaf_kernel<T, W>), dim3(1), dim3(LU_THREADS), 0, s, A, lda, m, c0, w, ipiv, info);
+    if (w > 0 && ncols > w) {
+        const int grid = (ncols + SWAP_THREADS / 32 - 1) / (SWAP_THREADS / 32);
+        hipLaunchKernelGGL((getrf_swap_kernel<W>), dim3(grid), dim3(SWAP_THREADS), 0, s, A, lda, ncols, c0, w, ipiv);
+    }
+    return (int)hipGetLastError();
+}
+
+struct Lu {
+    double *A;
+    int64_t lda;
+    int m, n;
+    int *ipiv, *info;
+    double *inv_ws;
+    void *gemm_ws;
+    size_t gemm_ws_bytes;
+    int cus;
+    hipStream_t s;
+    int err = 0;
+
+    double *at(int r, int c) const { return A + r + (int64_t)c * lda; }
+
+    // widest leaf the register budget allows for a panel of `rows` rows
+    static int leaf_width(int rows) {
+        return rows <= LU_THREADS ? 16 : rows <= 2 * LU_THREADS ? 16 : rows <= 4 * LU_THREADS ? 8
+               : rows <= 8 * LU_THREADS ? 4 : 2;
+    }
+
+    void run_leaf(int c0, int w) {
+        if (err) return;
+        const int rows = m - c0;
+        if (rows <= LU_THREADS) err = leaf<1, 16>(A, lda, m, n, c0, w, ipiv, info, s);
+        else if (rows <= 2 * LU_THREADS) err = leaf<2, 16>(A, lda, m, n, c0, w, ipiv, info, s);
+        else if (rows <= 4 * LU_THREADS) err = leaf<4, 8>(A, lda, m, n, c0, w, ipiv, info, s);
+        else if (rows <= 8 * LU_THREADS) err = leaf<8, 4>(A, lda, m, n, c0, w, ipiv, info, s);
+        else err = leaf<16, 2>(A, lda, m, n, c0, w, ipiv, info, s);
+    }
+
+    void gemm(int mm, int nn, int kk, const double *a, const double *b, double *c) {
+        if (err || mm <= 0 || nn <= 0 || kk <= 0) return;
+        GemmConfig cfg = pick_gemm_config(DT_F64, false, false, mm, nn, kk, (int)(lda - mm), cus);
+        if (gemm_workspace_bytes(mm, nn, kk, cfg, 8) > gemm_ws_bytes) cfg.splitk = 1;
+        err = launch_dgemm(false, false, mm, nn, kk, -1.0, a, lda, b, lda, 1.0, c, lda, cfg, gemm_ws, s);
+    }
+
+    // L11^-1 (unit lower, n1 x n1 at (c0, c0)) applied to the n1 x nn block at (c0, c1)
+    void trsm(int c0, int n1, int c1, int nn) {
+        if (err || nn <= 0) return;
+        if (n1 <= 16) {
+            hipLaunchKernelGGL(getrf_trsm_small_kernel<16>, dim3((nn + 63) / 64), dim3(64), 0, s, at(c0, c0), lda,
+                               at(c0, c1), n1, nn);
+            err = (int)hipGetLastError();
+            return;
+        }
+        if (n1 <= 32) {   // two substitutions around one GEMM beat the blocked solve's inverse pass here
+            const int h = n1 / 2;
+            trsm(c0, h, c1, nn);
+            gemm(n1 - h, nn, h, at(c0 + h, c0), at(c0, c1), at(c0 + h, c1));
+            trsm(c0 + h, n1 - h, c1, nn);
+            return;
+        }
+        err = launch_dtrsm(true, false, false, true, n1, nn, 1.0, at(c0, c0), lda, at(c0, c1), lda, inv_ws, gemm_ws,
+                           gemm_ws_bytes, cus, s);
+    }
+
+    // Factor columns [c0, c0 + nw) over rows [c0, m); columns right of the
+    // range are updated by the caller.
+    void rec(int c0, int nw) {
+        if (err || nw <= 0) return;
+        if (nw <= leaf_width(m - c0)) {
+            run_leaf(c0, nw);
+            return;
+        }
+        const int n1 = nw / 2, n2 = nw - n1;
+        rec(c0, n1);
+        trsm(c0, n1, c0 + n1, n2);
+        gemm(m - c0 - n1, n2, n1, at(c0 + n1, c0), at(c0, c0 + n1), at(c0 + n1, c0 + n1));
+        rec(c0 + n1, n2);
+    }
+};
+
+}  // namespace
+
+size_t getrf_gemm_workspace_bytes(int m, int n) {
+    // split-K partials of the widest Schur update: at most 8 partitions of m x n/2
+    return (size_t)8 * (size_t)(m > 0 ? m : 1) * (size_t)((n + 1) / 2 + 1) * sizeof(double);
+}
+
+int launch_dgetrf(int m, int n, double *A, int64_t lda, int *ipiv, int *info, double *inv_ws, void *gemm_ws,
+                  size_t gemm_ws_bytes, int cus, hipStream_t s) {
+    if (m > GETRF_MAX_ROWS) return (int)hipErrorInvalidValue;
+    Lu lu{A, lda, m, n, ipiv, info, inv_ws, gemm_ws, gemm_ws_bytes, cus, s};
+    const int mn = m < n ? m : n;
+    if (mn == 0) lu.run_leaf(0, 0);   // writes info = 0 and nothing else
+    lu.rec(0, mn);
+    // columns right of a wide matrix's square part: U12 = L11^-1 A12
+    if (n > m) lu.trsm(0, m, m, n - m);
+    return lu.err;
+}
+
+}  // namespace gemm_amd

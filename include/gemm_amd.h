@@ -64,6 +64,15 @@ void zgemm3m_(const char *transa, const char *transb, const int *m, const int *n
               const double *alpha, const double *a, const int *lda, const double *b, const int *ldb,
               const double *beta, double *c, const int *ldc);
 
+/* P*A = L*U with partial pivoting, LAPACK dgetrf_ contract: column-major m x n
+ * A overwritten by L (unit lower, below the diagonal) and U; ipiv[i] (1-based,
+ * min(m,n) entries) = row interchanged with row i; info = 0, -i for an illegal
+ * argument i, or i > 0 when U(i,i) is exactly zero (the factorisation is still
+ * completed).  No counterpart in the reference, which has no LAPACK layer: it
+ * is the first caller of the DTRSM and DGEMM paths above, which carry all but
+ * the panel columns.  m <= 16384. */
+void dgetrf_(const int *m, const int *n, double *a, const int *lda, int *ipiv, int *info);
+
 /* ===================================================== Part 2: device pointers */
 
 int gemm_amd_cgemm3m(char transa, char transb, int m, int n, int k, const float *alpha,
@@ -85,6 +94,11 @@ int gemm_amd_dtrsm(char side, char uplo, char transa, char diag, int m, int n, d
  * sb (L) or sa (R). */
 int gemm_amd_dtrsm_kernel(int variant, long M, long N, double *sa, double *sb, double *c, long LDB, long LDC,
                           void *stream);
+
+/* Device-pointer dgetrf_: a, ipiv and info in device memory, stream-ordered.
+ * Returns 0, the index of an illegal argument (m, n, -, lda = 1, 2, -, 4), or a
+ * negative HIP error (m > 16384 gives -hipErrorInvalidValue). */
+int gemm_amd_dgetrf(int m, int n, double *a, int lda, int *ipiv, int *info, void *stream);
 
 int gemm_amd_sgemm(char transa, char transb, int m, int n, int k, float alpha,
                    const float *a, int lda, const float *b, int ldb, float beta,

@@ -6,6 +6,7 @@
  */
 #include "oracle.h"
 #include <math.h>
+#include <float.h>
 #include <string.h>
 #include <stdlib.h>
 #include <quadmath.h>
@@ -497,4 +498,50 @@ void oracle_dgemm_blocked(char ta, char tb, int m, int n, int k, double alpha,
         free(ap);
         free(bp);
     }
+}
+
+/* ------------------------------------------------------------------ LU */
+/* Unblocked right-looking LU with partial pivoting, step for step LAPACK's
+ * dgetf2: idamax pivot (first maximum of |a|), full-row interchange, column
+ * scaled by the reciprocal pivot when |pivot| >= DBL_MIN (division otherwise),
+ * rank-1 update of the trailing block.  A zero pivot records info and skips
+ * the scaling, as dgetf2 does. */
+int oracle_dgetrf(int m, int n, double *a, int lda, int *ipiv) {
+    int info = 0;
+    const int mn = m < n ? m : n;
+    for (int j = 0; j < mn; ++j) {
+        int p = j;
+        double best = fabs(a[j + (int64_t)j * lda]);
+        for (int i = j + 1; i < m; ++i) {
+            const double v = fabs(a[i + (int64_t)j * lda]);
+            if (v > best) {
+                best = v;
+                p = i;
+            }
+        }
+        ipiv[j] = p + 1;
+        const double piv = a[p + (int64_t)j * lda];
+        if (piv != 0.0) {
+            if (p != j)
+                for (int c = 0; c < n; ++c) {
+                    const double t = a[j + (int64_t)c * lda];
+                    a[j + (int64_t)c * lda] = a[p + (int64_t)c * lda];
+                    a[p + (int64_t)c * lda] = t;
+                }
+            if (fabs(piv) >= DBL_MIN) {
+                const double r = 1.0 / piv;
+                for (int i = j + 1; i < m; ++i) a[i + (int64_t)j * lda] *= r;
+            } else {
+                for (int i = j + 1; i < m; ++i) a[i + (int64_t)j * lda] /= piv;
+            }
+        } else if (info == 0) {
+            info = j + 1;
+        }
+        for (int c = j + 1; c < n; ++c) {
+            const double u = a[j + (int64_t)c * lda];
+            if (u == 0.0) continue;
+            for (int i = j + 1; i < m; ++i) a[i + (int64_t)c * lda] -= a[i + (int64_t)j * lda] * u;
+        }
+    }
+    return info;
 }

@@ -83,6 +83,12 @@ void oracle_dgemm_blocked(char transa, char transb, int m, int n, int k, double 
                           const double *a, int lda, const double *b, int ldb,
                           double beta, double *c, int ldc, int nthreads);
 
+/* ---- LU with partial pivoting (LAPACK dgetf2 semantics; the reference has no
+ *      LU, so this is the netlib contract): column-major m x n a overwritten by
+ *      L (unit, below the diagonal) and U; ipiv 1-based; returns info (first
+ *      zero pivot column, 1-based, or 0).  Pivot = first row of maximal |a|. ---- */
+int oracle_dgetrf(int m, int n, double *a, int lda, int *ipiv);
+
 /* ---- configuration picker's perceptron, in double (mlp_oracle.c) ---- */
 #define ORACLE_MLP_IN 9
 #define ORACLE_MLP_HID 64

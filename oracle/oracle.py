@@ -48,6 +48,7 @@ def lib():
                                       _vp]),
             "oracle_dtrsm": (None, [_c, _c, _c, _c, _i, _i, _d, _vp, _i, _vp, _i]),
             "oracle_dtrsm_kernel": (None, [_i, _l, _l, _vp, _vp, _vp, _l, _l]),
+            "oracle_dgetrf": (_i, [_i, _i, _vp, _i, _vp]),
             "oracle_zgemm": (None, [_c, _c, _i, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _vp, _i]),
             "oracle_cgemm": (None, [_c, _c, _i, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _vp, _i]),
             "oracle_err_f64": (None, [_i64, _vp, _vp, _vp, _vp]),
@@ -149,6 +150,15 @@ def dtrsm(side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb):
 
 def dtrsm_kernel(variant, M, N, sa, sb, C, LDB, LDC):
     lib().oracle_dtrsm_kernel(variant, M, N, _p(sa), _p(sb), _p(C), LDB, LDC)
+
+
+def dgetrf(m, n, a, lda):
+    """LU with partial pivoting in place on a Fortran-order float64 array;
+    returns (ipiv int32 1-based, info)."""
+    assert a.dtype == np.float64 and a.flags.f_contiguous
+    ipiv = np.zeros(max(min(m, n), 1), dtype=np.int32)
+    info = lib().oracle_dgetrf(m, n, _p(a), lda, _p(ipiv))
+    return ipiv[: min(m, n)], info
 
 
 def zgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
