@@ -33,6 +33,18 @@ SIGNATURES = {
     "dgemm_": (None, [c_vp] * 13),
     "dtrsm_": (None, [c_vp] * 11),
     "dgetrf_": (None, [c_vp] * 6),
+    # CBLAS (enum arguments as int) and ILP64 Fortran symbols
+    "cblas_sgemm": (None, [c_int] * 6 + [c_float, c_vp, c_int, c_vp, c_int, c_float, c_vp, c_int]),
+    "cblas_dgemm": (None, [c_int] * 6 + [c_double, c_vp, c_int, c_vp, c_int, c_double, c_vp, c_int]),
+    "cblas_cgemm3m": (None, [c_int] * 6 + [c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_int]),
+    "cblas_zgemm3m": (None, [c_int] * 6 + [c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_int]),
+    "cblas_dtrsm": (None, [c_int] * 7 + [c_double, c_vp, c_int, c_vp, c_int]),
+    "sgemm_64_": (None, [c_vp] * 13),
+    "dgemm_64_": (None, [c_vp] * 13),
+    "cgemm3m_64_": (None, [c_vp] * 13),
+    "zgemm3m_64_": (None, [c_vp] * 13),
+    "dtrsm_64_": (None, [c_vp] * 11),
+    "dgetrf_64_": (None, [c_vp] * 6),
     "cgemm3m_": (None, [c_vp] * 13),
     "zgemm3m_": (None, [c_vp] * 13),
     "gemm_amd_cgemm3m": (c_int, [c_char, c_char, c_int, c_int, c_int, c_vp, c_vp, c_int, c_vp, c_int, c_vp, c_vp,

@@ -73,6 +73,42 @@ void zgemm3m_(const char *transa, const char *transb, const int *m, const int *n
  * the panel columns.  m <= 16384. */
 void dgetrf_(const int *m, const int *n, double *a, const int *lda, int *ipiv, int *info);
 
+/* ----------------------------------------------------- Part 1b: CBLAS and ILP64
+ * Wrappers over the routines above with no counterpart in the reference,
+ * which exports only the Fortran LP64 symbols (src/gemm_driver.c:116): the
+ * CBLAS interface for C callers (row-major or column-major; the enum values
+ * of cblas.h: RowMajor 101, ColMajor 102, NoTrans 111, Trans 112, ConjTrans
+ * 113, Upper 121, Lower 122, NonUnit 131, Unit 132, Left 141, Right 142;
+ * illegal arguments are reported with CBLAS numbering) and the ILP64 Fortran
+ * symbols (64-bit integers, suffix 64_, as OpenBLAS names them), which reject
+ * values outside the 32-bit range the kernels index with.  Host pointers. */
+void cblas_sgemm(int layout, int transa, int transb, int m, int n, int k, float alpha, const float *a, int lda,
+                 const float *b, int ldb, float beta, float *c, int ldc);
+void cblas_dgemm(int layout, int transa, int transb, int m, int n, int k, double alpha, const double *a, int lda,
+                 const double *b, int ldb, double beta, double *c, int ldc);
+void cblas_cgemm3m(int layout, int transa, int transb, int m, int n, int k, const void *alpha, const void *a,
+                   int lda, const void *b, int ldb, const void *beta, void *c, int ldc);
+void cblas_zgemm3m(int layout, int transa, int transb, int m, int n, int k, const void *alpha, const void *a,
+                   int lda, const void *b, int ldb, const void *beta, void *c, int ldc);
+void cblas_dtrsm(int layout, int side, int uplo, int transa, int diag, int m, int n, double alpha, const double *a,
+                 int lda, double *b, int ldb);
+void sgemm_64_(const char *transa, const char *transb, const int64_t *m, const int64_t *n, const int64_t *k,
+               const float *alpha, const float *a, const int64_t *lda, const float *b, const int64_t *ldb,
+               const float *beta, float *c, const int64_t *ldc);
+void dgemm_64_(const char *transa, const char *transb, const int64_t *m, const int64_t *n, const int64_t *k,
+               const double *alpha, const double *a, const int64_t *lda, const double *b, const int64_t *ldb,
+               const double *beta, double *c, const int64_t *ldc);
+void cgemm3m_64_(const char *transa, const char *transb, const int64_t *m, const int64_t *n, const int64_t *k,
+                 const float *alpha, const float *a, const int64_t *lda, const float *b, const int64_t *ldb,
+                 const float *beta, float *c, const int64_t *ldc);
+void zgemm3m_64_(const char *transa, const char *transb, const int64_t *m, const int64_t *n, const int64_t *k,
+                 const double *alpha, const double *a, const int64_t *lda, const double *b, const int64_t *ldb,
+                 const double *beta, double *c, const int64_t *ldc);
+void dtrsm_64_(const char *side, const char *uplo, const char *transa, const char *diag, const int64_t *m,
+               const int64_t *n, const double *alpha, const double *a, const int64_t *lda, double *b,
+               const int64_t *ldb);
+void dgetrf_64_(const int64_t *m, const int64_t *n, double *a, const int64_t *lda, int64_t *ipiv, int64_t *info);
+
 /* ===================================================== Part 2: device pointers */
 
 int gemm_amd_cgemm3m(char transa, char transb, int m, int n, int k, const float *alpha,
