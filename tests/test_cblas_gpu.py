@@ -52,7 +52,7 @@ def test_cblas_gemm_row_major(gpu_lib, ta, tb, dt):
     A_full, A = rowmajor_padded(rng, *((m, k) if ta == "N" else (k, m)), pad, dt)
     B_full, B = rowmajor_padded(rng, *((k, n) if tb == "N" else (n, k)), pad, dt)
     C_full, C0 = rowmajor_padded(rng, m, n, pad, dt)
-    C_full = np.ascontiguousarray(C_full)
+    C0 = C0.copy()   # the call overwrites C_full, of which C0 is a view
     c_ref = 1.5 * op(A.astype(np.float64), ta) @ op(B.astype(np.float64), tb) + 0.5 * C0.astype(np.float64)
     fn = _lib.lib().cblas_dgemm if dt == np.float64 else _lib.lib().cblas_sgemm
     fn(ROW, TR[ta], TR[tb], m, n, k, 1.5, P(A_full), A_full.shape[1], P(B_full), B_full.shape[1], 0.5, P(C_full),
