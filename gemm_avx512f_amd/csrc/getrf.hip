@@ -58,7 +58,11 @@ __global__ __launch_bounds__(LU_THREADS) void getrf_leaf_kernel(double *__restri
         for (int k = 0; k < W; ++k) v[t][k] = (r < m && k < w) ? A[r + (int64_t)(c0 + k) * lda] : 0.0;
     }
 
-    for (int j = 0; j < w; ++j) {
+    // Unrolled over the panel width so that the column index is a constant:
+    // the register panel is then addressed directly, without select chains.
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        if (j >= w) continue;   // w is uniform: every thread skips the same columns
         const int cj = c0 + j, b = j & 1;
         // 1. row cj's entries; pivot candidates over rows >= cj (first maximum of |a|)
         double best = -1.0;
