@@ -4,7 +4,7 @@
 // The factorisation is recursive over columns (left half, then the U12 solve
 // and the Schur-complement GEMM, then the right half), so all but the leaf
 // columns run in the DTRSM and FP64 matrix-core GEMM kernels of this library.
-// A leaf is a panel of up to 16 columns and every row below its diagonal,
+// A leaf is a panel of up to 32 columns and every row below its diagonal,
 // factored by one 1024-thread workgroup with the panel held in registers
 // (rows strided over threads).  After each leaf a chip-wide kernel applies
 // the leaf's row interchanges to every other column of the matrix, so the
@@ -170,10 +170,13 @@ __global__ __launch_bounds__(LU_THREADS) void getrf_leaf_kernel(double *__restri
 // Applies the interchanges ipiv[c0 .. c0+w) (1-based rows, in order) to every
 // column in [0, ncols) outside [c0, c0+w).  Each workgroup first reduces the
 // w swaps to their net effect on the rows they touch (the value that ends at
-// row dst started at row src; at most 2w rows move), then 32 lanes move one
-// column: every lane of a column loads its source before any lane of that
-// column stores (one instruction each).
+// row dst started at row src; at most 2w rows move), then 32 or 64 lanes (one
+// per move) move one column: every lane of a column loads its source before
+// any lane of that column stores (one instruction each, one wave per column
+// group).
 constexpr int SWAP_THREADS = 256;
+// lanes that move one column: one per row move (at most 2w), 32 or 64
+constexpr int swap_lanes_per_column(int w) { return 2 * w > 32 ? 64 : 32; }
 
 template <int W>
 __global__ __launch_bounds__(SWAP_THREADS) void getrf_swap_kernel(double *__restrict__ A, int64_t lda, int ncols,
@@ -225,8 +228,9 @@ __global__ __launch_bounds__(SWAP_THREADS) void getrf_swap_kernel(double *__rest
         if (lane == 0) s_npairs = __popcll(mask);
     }
     __syncthreads();
-    const int np = s_npairs, pair = tid & 31;
-    const int c = blockIdx.x * (SWAP_THREADS / 32) + (tid >> 5);
+    constexpr int LPC = swap_lanes_per_column(W);
+    const int np = s_npairs, pair = tid % LPC;
+    const int c = blockIdx.x * (SWAP_THREADS / LPC) + tid / LPC;
     if (pair >= np || c >= ncols || (c >= c0 && c < c0 + w)) return;
     double *col = A + (int64_t)c * lda;
     const double x = col[s_src[pair]];
@@ -267,7 +271,8 @@ template <int T, int W>
 int leaf(double *A, int64_t lda, int m, int ncols, int c0, int w, int *ipiv, int *info, hipStream_t s) {
     hipLaunchKernelGGL((getrf_leaf_kernel<T, W>), dim3(1), dim3(LU_THREADS), 0, s, A, lda, m, c0, w, ipiv, info);
     if (w > 0 && ncols > w) {
-        const int grid = (ncols + SWAP_THREADS / 32 - 1) / (SWAP_THREADS / 32);
+        constexpr int cols_per_block = SWAP_THREADS / swap_lanes_per_column(W);
+        const int grid = (ncols + cols_per_block - 1) / cols_per_block;
         hipLaunchKernelGGL((getrf_swap_kernel<W>), dim3(grid), dim3(SWAP_THREADS), 0, s, A, lda, ncols, c0, w, ipiv);
     }
     return (int)hipGetLastError();
@@ -289,14 +294,14 @@ struct Lu {
 
     // widest leaf the register budget allows for a panel of `rows` rows
     static int leaf_width(int rows) {
-        return rows <= LU_THREADS ? 16 : rows <= 2 * LU_THREADS ? 16 : rows <= 4 * LU_THREADS ? 8
+        return rows <= LU_THREADS ? 32 : rows <= 2 * LU_THREADS ? 16 : rows <= 4 * LU_THREADS ? 8
                : rows <= 8 * LU_THREADS ? 4 : 2;
     }
 
     void run_leaf(int c0, int w) {
         if (err) return;
         const int rows = m - c0;
-        if (rows <= LU_THREADS) err = leaf<1, 16>(A, lda, m, n, c0, w, ipiv, info, s);
+        if (rows <= LU_THREADS) err = leaf<1, 32>(A, lda, m, n, c0, w, ipiv, info, s);
         else if (rows <= 2 * LU_THREADS) err = leaf<2, 16>(A, lda, m, n, c0, w, ipiv, info, s);
         else if (rows <= 4 * LU_THREADS) err = leaf<4, 8>(A, lda, m, n, c0, w, ipiv, info, s);
         else if (rows <= 8 * LU_THREADS) err = leaf<8, 4>(A, lda, m, n, c0, w, ipiv, info, s);
