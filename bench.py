@@ -8,9 +8,14 @@ device).  W untimed warm-up steps, then exactly K steps between two HIP
 events on the launch stream, with a device synchronize on both sides.
 
 Prints one JSON line.  `roofline` prices the dominant kernel against the
-FP64 matrix-core peak (78.6 TFLOP/s); `cpu_baseline` times the reference's
-own CPU dgemm_ (built from its sources into oracle/_ref) on this host's cores
-on the same workload, when the host has AVX-512F.
+FP64 matrix-core peak (78.6 TFLOP/s); `emulated` is the fastest int8
+emulation whose sampled max error is no larger than native FP64's on the
+headline inputs and on inputs of every dynamic range r in {0, 8, 32};
+`dropin_host_pointer` is the reference's own entry point dgemm_ on host
+arrays, PCIe copies included; `cpu_baseline` times the reference's own CPU
+dgemm_ (built from its sources into oracle/_ref) on all physical cores of
+this host, one OpenMP thread per core with close binding, on the same
+workload, when the host has AVX-512F.
 """
 import argparse
 import json
@@ -82,6 +87,7 @@ def gpu_leg(size, steps, warmup, variants=(("crt", 16), ("slices", 8)), seed=0):
     if not np.isfinite(c_nat).all():
         raise RuntimeError("non-finite DGEMM result")
     out["native"]["err"] = _sampled_error(a, b, c_nat, rows, cols)
+    out["host_ab"] = (a, b)
     del c_nat
     out["emulated"] = []
     for kind, count in variants:
@@ -98,6 +104,7 @@ def gpu_leg(size, steps, warmup, variants=(("crt", 16), ("slices", 8)), seed=0):
             path = "gemm_amd_dgemm_ozaki (int8 slices, %d slices, %d int8 GEMMs)" % (count, products)
         ms2, wall2 = _time(fn, stream, steps, warmup)
         out["emulated"].append({"kind": kind, "count": count, "products": products, "path": path, "ms": ms2,
+                                "accuracy_key": ("crt_%d_moduli" if kind == "crt" else "slices_%d") % count,
                                 "wall": wall2, "err": _sampled_error(a, b, C2.to_host(), rows, cols)})
         del C2
     return out
@@ -344,16 +351,45 @@ def grouped_leg(replays=20):
     return out
 
 
-def cpu_leg(size, reps=3):
-    """Reference dgemm_ (oracle/_ref/DGEMM.so, -O2 -mavx512f -fopenmp) on this host."""
+def physical_cores():
+    """Physical cores this process may run on: distinct (package, core) pairs
+    of /proc/cpuinfo among the CPUs in the affinity mask, capped by the cgroup
+    CPU quota (cpu.max) when one is set."""
+    allowed = os.sched_getaffinity(0)
+    cores, cpu, pkg = set(), None, "0"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "processor":
+                    cpu = int(val)
+                elif key == "physical id":
+                    pkg = val
+                elif key == "core id" and cpu in allowed:
+                    cores.add((pkg, val))
+    except OSError:
+        pass
+    n = len(cores) or len(allowed)
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()
+        if quota != "max":
+            n = min(n, max(1, int(int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def cpu_leg_child(size, reps):
+    """Body of the CPU leg (runs in a child process whose OpenMP environment
+    was set before the runtime loaded): the reference's dgemm_
+    (oracle/_ref/DGEMM.so, -O2 -mavx512f -fopenmp) on this host, or the
+    blocked C port of its algorithm where the host lacks AVX-512F."""
     from oracle import oracle as O
 
-    threads = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     threads = int(os.environ["OMP_NUM_THREADS"])
-    kind = "reference"
-    if O.ref_lib("DGEMM.so") is None:
-        kind = "port"
+    kind = "reference" if O.ref_lib("DGEMM.so") is not None else "port"
     m = n = k = size
     s = O.Stream(0)
     a, b, c = s.f64(m, k), s.f64(k, n), s.f64(m, n)
@@ -367,9 +403,51 @@ def cpu_leg(size, reps=3):
         dt = time.perf_counter() - t0
         if i > 0:
             best = dt if best is None else min(best, dt)
-    tflops = 2.0 * m * n * k / best / 1e12
-    return {"value": round(tflops, 4), "unit": "TFLOP/s", "cores": threads, "kind": kind,
-            "sample": "DGEMM %d^3 NN alpha=1 beta=0, best of %d after 1 warm-up (%.2f s per call)" % (size, reps, best)}
+    return {"value": round(2.0 * m * n * k / best / 1e12, 4), "unit": "TFLOP/s", "cores": threads, "kind": kind,
+            "sample": "DGEMM %d^3 NN alpha=1 beta=0, seed-0 splitmix64 inputs, best of %d calls after 1 warm-up "
+                      "(%.2f s per call), one OpenMP thread per physical core, OMP_PROC_BIND=close "
+                      "OMP_PLACES=cores" % (size, reps, best)}
+
+
+def cpu_leg(size, reps=3):
+    """The reference's CPU dgemm_ timed on all physical cores of this host
+    (counted by physical_cores), in a child process so that OMP_NUM_THREADS,
+    OMP_PROC_BIND=close and OMP_PLACES=cores are in place before the OpenMP
+    runtime loads."""
+    import subprocess
+
+    cores = physical_cores()
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg-child", str(size), str(reps)],
+                         env=env, capture_output=True, text=True, timeout=900)
+    if out.returncode != 0:
+        raise RuntimeError("CPU leg failed: " + out.stderr[-2000:])
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    res["host_logical_cpus"] = os.cpu_count()
+    return res
+
+
+def dropin_leg(size, a, b, reps=3):
+    """The host-pointer drop-in dgemm_ (the reference's own entry point,
+    src/gemm_driver.c:116) at the headline size: copies of A and B into HBM,
+    the same kernel, the copy of C back; wall time per call, best of `reps`
+    after one warm-up call."""
+    from gemm_avx512f_amd import blas
+
+    m = n = k = size
+    c = np.zeros((m, n), order="F")
+    best = None
+    for i in range(reps + 1):
+        t0 = time.perf_counter()
+        blas.dgemm_("N", "N", m, n, k, 1.0, a, m, b, k, 0.0, c, m)
+        dt = time.perf_counter() - t0
+        if i > 0:
+            best = dt if best is None else min(best, dt)
+    gib = 3.0 * 8 * size * size / 2 ** 30
+    return {"ms_per_call": round(best * 1e3, 2), "tflops_incl_pcie": round(2.0 * m * n * k / best / 1e12, 3),
+            "host_bytes_moved_gib": round(gib, 3),
+            "note": "dgemm_ on host (pageable) arrays: H2D of A and B, kernel, D2H of C (beta = 0 skips the H2D "
+                    "of C); wall clock, best of %d after 1 warm-up" % reps}
 
 
 def main():
@@ -388,10 +466,16 @@ def main():
     ap.add_argument("--no-sgemm", action="store_true", help="skip the SGEMM / bf16x split leg")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the SGEMM 1024^3 / DGEMM 4096^3 / DTRSM 4096x512 leg")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the host-pointer dgemm_ leg (PCIe copies included)")
+    ap.add_argument("--cpu-leg-child", nargs=2, type=int, metavar=("SIZE", "REPS"), help=argparse.SUPPRESS)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB first (a known-size kernel that "
                          "calibrates the HBM byte counters when the run is profiled)")
     args = ap.parse_args()
+    if args.cpu_leg_child:
+        print(json.dumps(cpu_leg_child(*args.cpu_leg_child)), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the DGEMM on a single MI355X "
               "(multi-GPU sharding is a design note, not a measured path); rerun with --gpus 1", file=sys.stderr)
@@ -441,6 +525,15 @@ def main():
                        "sample": "256 entries vs long-double dot products"},
         "wall_s_timed_region": round(wall, 3),
     }
+    from gemm_avx512f_amd import _lib
+    import ctypes
+    tile, splitk = ctypes.c_int(), ctypes.c_int()
+    _lib.check(_lib.lib().gemm_amd_pick_config(1, b"N", b"N", size, size, size, 0, 2, ctypes.byref(tile),
+                                               ctypes.byref(splitk)), "pick_config")
+    result["config"]["kernel_config"] = {"tile": ["128x128", "64x64", "128x64", "256x64"][tile.value],
+                                         "splitk": splitk.value,
+                                         "kernels_per_step": 1 if splitk.value == 1 else 2}
+    accuracy = accuracy_leg(size) if not args.no_emulated else None
     variants = []
     for e in legs["emulated"]:
         em_tf = flops * args.steps / (e["ms"] / 1e3) / 1e12
@@ -452,16 +545,27 @@ def main():
             "int8_frac_of_peak": round(em_tf * e["products"] / PEAK_INT8_TOPS, 4),
             "ms_per_step": round(e["ms"] / args.steps, 4),
             "max_rel": e["err"][0], "rms_rel": e["err"][1],
-            "fp64_level": bool(e["err"][0] <= legs["native"]["err"][0]),
-            "faster_than_native": bool(em_tf > tflops),
         })
+        # FP64-level: max error no larger than native FP64's on the headline
+        # inputs and on every input range of the accuracy leg
+        at = {"uniform": bool(e["err"][0] <= legs["native"]["err"][0])}
+        for rk in ("r=0", "r=8", "r=32"):
+            row = accuracy[rk]
+            at[rk] = bool(row[e["accuracy_key"]]["max_rel"] <= row["native_fp64"]["max_rel"])
+        variants[-1]["fp64_level_at"] = at
+        variants[-1]["fp64_level"] = all(at.values())
+        variants[-1]["faster_than_native"] = bool(em_tf > tflops)
     if variants:
-        # headline emulated figure: the fastest variant whose error is at FP64 level
-        ok = [v for v in variants if v["fp64_level"]] or variants
-        result["emulated"] = max(ok, key=lambda v: v["tflops_fp64_equiv"])
+        # headline emulated figure: the fastest variant at FP64 level on every input range
+        ok = [v for v in variants if v["fp64_level"]]
+        result["emulated"] = max(ok, key=lambda v: v["tflops_fp64_equiv"]) if ok else None
         result["emulated_variants"] = variants
-    if not args.no_emulated:
-        result["accuracy_vs_range"] = accuracy_leg(size)
+    if accuracy is not None:
+        result["accuracy_vs_range"] = accuracy
+    if not args.no_dropin:
+        a, b = legs["host_ab"]
+        result["dropin_host_pointer"] = dropin_leg(size, a, b)
+    del legs["host_ab"]
     if not args.no_sgemm:
         result["sgemm"] = sgemm_leg(size, args.steps, args.warmup)
     if not args.no_grouped:

@@ -256,13 +256,7 @@ int launch_sgemm_bf16x(bool ta, bool tb, int m, int n, int k, float alpha, const
     p.ldc = ldc;
     constexpr size_t lds = 2ull * (BM + BN) * (KB + 16);
     auto kern = bf16x_gemm_kernel<WM, WN, TM, TN, KB>;
-    static bool attr = false;
-    if (!attr) {
-        hipError_t he =
-            hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (he != hipSuccess) return (int)he;
-        attr = true;
-    }
+    if (hipError_t he = lds_limit_once<bf16x_gemm_kernel<WM, WN, TM, TN, KB>>(lds)) return (int)he;
     hipLaunchKernelGGL(kern, dim3(p.tiles_m * p.tiles_n), dim3(WM * WN * 64), lds, s, p);
     return (int)hipGetLastError();
 }

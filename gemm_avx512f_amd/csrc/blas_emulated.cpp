@@ -25,9 +25,10 @@ extern "C" int gemm_amd_sgemm_bf16x(char ta, char tb, int m, int n, int k, float
     if (alpha == 0.0f || k == 0) return hip_status((hipError_t)launch_scale_s(m, n, beta, c, ldc, s), "SGEMM_BF16X");
     const int nsplit = nprod == 3 ? 2 : 3;
     int st = 0;
-    void *wa = device_workspace(WS_EMU_A, bf16x_workspace_bytes(m, k, nsplit), &st);
+    WsLease lease(s);
+    void *wa = lease.get(WS_EMU_A, bf16x_workspace_bytes(m, k, nsplit), &st);
     if (st) return st;
-    void *wb = device_workspace(WS_EMU_B, bf16x_workspace_bytes(n, k, nsplit), &st);
+    void *wb = lease.get(WS_EMU_B, bf16x_workspace_bytes(n, k, nsplit), &st);
     if (st) return st;
     const int e = launch_sgemm_bf16x(!notrans(ta), !notrans(tb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nprod,
                                      wa, wb, s);
@@ -66,11 +67,12 @@ extern "C" int gemm_amd_dgemm_ozaki(char ta, char tb, int m, int n, int k, doubl
     hipStream_t s = (hipStream_t)stream;
     if (alpha == 0.0 || k == 0) return hip_status((hipError_t)launch_scale_d(m, n, beta, c, ldc, s), "DGEMM_OZAKI");
     int st = 0;
-    void *wa = device_workspace(WS_EMU_A, ozaki_workspace_bytes(m, k, nslices), &st);
+    WsLease lease(s);
+    void *wa = lease.get(WS_EMU_A, ozaki_workspace_bytes(m, k, nslices), &st);
     if (st) return st;
-    void *wb = device_workspace(WS_EMU_B, ozaki_workspace_bytes(n, k, nslices), &st);
+    void *wb = lease.get(WS_EMU_B, ozaki_workspace_bytes(n, k, nslices), &st);
     if (st) return st;
-    void *ww = device_workspace(WS_EMU_W, ozaki_accum_bytes(m, n), &st);
+    void *ww = lease.get(WS_EMU_W, ozaki_accum_bytes(m, n), &st);
     if (st) return st;
     const int e = launch_dgemm_ozaki(!notrans(ta), !notrans(tb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nslices,
                                      wa, wb, ww, s);
@@ -88,7 +90,8 @@ extern "C" int gemm_amd_ozaki_split(const double *x, int64_t row_stride, int64_t
     if (rows == 0 || k == 0) return 0;
     if (int st = init_status()) return st;
     int st = 0;
-    void *rm = device_workspace(WS_EMU_AUX, (size_t)rows_pad * 8, &st);
+    WsLease lease((hipStream_t)stream);
+    void *rm = lease.get(WS_EMU_AUX, (size_t)rows_pad * 8, &st);
     if (st) return st;
     return hip_status((hipError_t)launch_ozaki_split_only(x, row_stride, k_stride, rows, k, nslices, slices, rows_pad,
                                                           k_pad, exps, (unsigned long long *)rm, (hipStream_t)stream),
@@ -117,11 +120,12 @@ extern "C" int gemm_amd_dgemm_ozaki_crt(char ta, char tb, int m, int n, int k, d
     if (alpha == 0.0 || k == 0)
         return hip_status((hipError_t)launch_scale_d(m, n, beta, c, ldc, s), "DGEMM_OZAKI_CRT");
     int st = 0;
-    void *wa = device_workspace(WS_EMU_A, ozaki_crt_workspace_bytes(m, k, nmoduli), &st);
+    WsLease lease(s);
+    void *wa = lease.get(WS_EMU_A, ozaki_crt_workspace_bytes(m, k, nmoduli), &st);
     if (st) return st;
-    void *wb = device_workspace(WS_EMU_B, ozaki_crt_workspace_bytes(n, k, nmoduli), &st);
+    void *wb = lease.get(WS_EMU_B, ozaki_crt_workspace_bytes(n, k, nmoduli), &st);
     if (st) return st;
-    void *ww = device_workspace(WS_EMU_W, ozaki_crt_accum_bytes(m, n, nmoduli), &st);
+    void *ww = lease.get(WS_EMU_W, ozaki_crt_accum_bytes(m, n, nmoduli), &st);
     if (st) return st;
     const int e = launch_dgemm_ozaki_crt(!notrans(ta), !notrans(tb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
                                          nmoduli, wa, wb, ww, s);
@@ -145,7 +149,8 @@ extern "C" int gemm_amd_ozaki_crt_split(const double *x, int64_t row_stride, int
     if (rows == 0 || k == 0) return 0;
     if (int st = init_status()) return st;
     int st = 0;
-    void *rm = device_workspace(WS_EMU_AUX, (size_t)rows_pad * 8, &st);
+    WsLease lease((hipStream_t)stream);
+    void *rm = lease.get(WS_EMU_AUX, (size_t)rows_pad * 8, &st);
     if (st) return st;
     return hip_status((hipError_t)launch_ozaki_crt_split_only(x, row_stride, k_stride, rows, k, nmoduli, bits,
                                                               residues, rows_pad, k_pad, exps,

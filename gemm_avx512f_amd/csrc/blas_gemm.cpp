@@ -39,12 +39,14 @@ int gemm_device(const char *name, char ta, char tb, int m, int n, int k, T alpha
     GemmConfig cfg = force ? *force : pick_gemm_config(sizeof(T) == 8 ? DT_F64 : DT_F32, !notrans(ta),
                                                        !notrans(tb), m, n, k, lda - (notrans(ta) ? m : k), rt().cus);
     const size_t wbytes = gemm_workspace_bytes(m, n, k, cfg, (int)sizeof(T));
-    void *work = nullptr;
-    if (wbytes) {
-        int st = 0;
-        work = device_workspace(WS_GEMM, wbytes, &st);
-        if (st) return st;
-    }
+    if (!wbytes)
+        return hip_status((hipError_t)launch<T>(!notrans(ta), !notrans(tb), m, n, k, alpha, a, lda, b, ldb, beta, c,
+                                                ldc, cfg, nullptr, s),
+                          name);
+    WsLease lease(s);
+    int st = 0;
+    void *work = lease.get(WS_GEMM, wbytes, &st);
+    if (st) return st;
     int e = launch<T>(!notrans(ta), !notrans(tb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, cfg, work, s);
     return hip_status((hipError_t)e, name);
 }

@@ -15,13 +15,14 @@ template <typename T>
 int run3m(char ta, char tb, int m, int n, int k, const T *alpha, const T *a, int lda, const T *b, int ldb,
           const T *beta, T *c, int ldc, hipStream_t s, const GemmConfig *force = nullptr) {
     int st = 0;
-    void *ws = device_workspace(WS_COMPLEX, gemm3m_workspace_bytes(m, n, k, sizeof(T)), &st);
+    WsLease lease(s);
+    void *ws = lease.get(WS_COMPLEX, gemm3m_workspace_bytes(m, n, k, sizeof(T)), &st);
     if (st) return st;
     const int nrowa = notrans(ta) ? m : k;
     const GemmConfig cfg = force ? *force
                                  : pick_gemm_config(sizeof(T) == 8 ? DT_C64 : DT_C32, !notrans(ta), !notrans(tb), m,
                                                     n, k, lda - nrowa, rt().cus);
-    void *gws = device_workspace(WS_GEMM, gemm_workspace_bytes(m, n, k, cfg, (int)sizeof(T)), &st);
+    void *gws = lease.get(WS_GEMM, gemm_workspace_bytes(m, n, k, cfg, (int)sizeof(T)), &st);
     if (st) return st;
     int e;
     if constexpr (sizeof(T) == 8)

@@ -27,10 +27,11 @@ int getrf_run(int m, int n, double *a, int lda, int *ipiv, int *info, hipStream_
     }
     const int mn = m < n ? m : n;
     int st = 0;
-    double *inv = (double *)device_workspace(WS_TRSM_INV, trsm_workspace_inv_bytes(mn > 0 ? mn : 1), &st);
+    WsLease lease(s);
+    double *inv = (double *)lease.get(WS_TRSM_INV, trsm_workspace_inv_bytes(mn > 0 ? mn : 1), &st);
     if (st) return st;
     const size_t gbytes = getrf_gemm_workspace_bytes(m, n);
-    void *gws = device_workspace(WS_GEMM, gbytes, &st);
+    void *gws = lease.get(WS_GEMM, gbytes, &st);
     if (st) return st;
     const int e = launch_dgetrf(m, n, a, lda, ipiv, info, inv, gws, gbytes, rt().cus, s);
     return hip_status((hipError_t)e, "DGETRF");
@@ -60,7 +61,19 @@ void dgetrf_(const int *pm, const int *pn, double *a, const int *plda, int *ipiv
     }
     *info = 0;
     if (m == 0 || n == 0) return;
+    if (m > GETRF_MAX_ROWS) {
+        // a legal LAPACK argument this library does not support: reported as
+        // an illegal first argument, nothing factored
+        *info = -1;
+        fprintf(stderr, " ** On entry to DGETRF parameter number  1 had an unsupported value (m = %d > %d)\n", m,
+                GETRF_MAX_ROWS);
+        set_error("DGETRF: m exceeds the supported maximum of 16384 rows");
+        return;
+    }
+    // every failure below leaves info = GEMM_AMD_INFO_RUNTIME_ERROR (the
+    // matrix and ipiv are then not a factorisation)
     if (init_status()) {
+        *info = GEMM_AMD_INFO_RUNTIME_ERROR;
         fprintf(stderr, "DGETRF: %s\n", get_error());
         return;
     }
@@ -71,6 +84,7 @@ void dgetrf_(const int *pm, const int *pn, double *a, const int *plda, int *ipiv
     int st = buf(BUF_A).ensure((size_t)m * n * sizeof(double));
     if (!st) st = buf(BUF_WORK).ensure((size_t)(mn + 1) * sizeof(int));
     if (st) {
+        *info = GEMM_AMD_INFO_RUNTIME_ERROR;
         fprintf(stderr, "DGETRF: %s\n", get_error());
         return;
     }
@@ -79,10 +93,12 @@ void dgetrf_(const int *pm, const int *pn, double *a, const int *plda, int *ipiv
     hipError_t e = hipMemcpy2DAsync(dA, (size_t)m * 8, a, (size_t)lda * 8, (size_t)m * 8, n, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) {
         hip_status(e, "DGETRF upload");
+        *info = GEMM_AMD_INFO_RUNTIME_ERROR;
         fprintf(stderr, "DGETRF: %s\n", get_error());
         return;
     }
     if (getrf_run(m, n, dA, m, dpiv, dinfo, s)) {
+        *info = GEMM_AMD_INFO_RUNTIME_ERROR;
         fprintf(stderr, "DGETRF: %s\n", get_error());
         return;
     }
@@ -92,6 +108,7 @@ void dgetrf_(const int *pm, const int *pn, double *a, const int *plda, int *ipiv
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
         hip_status(e, "DGETRF download");
+        *info = GEMM_AMD_INFO_RUNTIME_ERROR;
         fprintf(stderr, "DGETRF: %s\n", get_error());
     }
 }

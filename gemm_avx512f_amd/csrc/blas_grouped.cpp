@@ -100,7 +100,8 @@ int gemm_amd_zgemm3m_batched(int count, const char *transa, const char *transb, 
     const size_t tab = (off[count] * sizeof(double) + 255) / 256 * 256;
     const size_t tabs = 2 * (size_t)count * sizeof(SplitJob3m), tabc = (size_t)count * sizeof(CombineJob3m);
     int st = 0;
-    char *wsb = (char *)device_workspace(WS_COMPLEX, tab + tabs + tabc + 256, &st);
+    WsLease lease(s);
+    char *wsb = (char *)lease.get(WS_COMPLEX, tab + tabs + tabc + 256, &st);
     if (st) return st;
     double *ws = (double *)wsb;
     SplitJob3m *d_split = (SplitJob3m *)(wsb + tab);

@@ -30,11 +30,12 @@ int trsm_run(char side, char uplo, char transa, char diag, int m, int n, double 
     const bool left = lsame(side, 'L');
     const int na = left ? m : n;
     int st = 0;
-    double *inv = (double *)device_workspace(WS_TRSM_INV, trsm_workspace_inv_bytes(na), &st);
+    WsLease lease(s);
+    double *inv = (double *)lease.get(WS_TRSM_INV, trsm_workspace_inv_bytes(na), &st);
     if (st) return st;
     // split-K partials of the widest trailing update: at most 8 partitions of an m x n (or smaller) panel
     const size_t gbytes = (size_t)8 * (size_t)m * (size_t)n * sizeof(double);
-    void *gws = device_workspace(WS_GEMM, gbytes, &st);
+    void *gws = lease.get(WS_GEMM, gbytes, &st);
     if (st) return st;
     const int e = launch_dtrsm(left, lsame(uplo, 'U'), !notrans(transa), lsame(diag, 'U'), m, n, alpha, a, lda, b,
                                ldb, inv, gws, gbytes, rt().cus, s);

@@ -83,13 +83,14 @@ int gemm_amd_tune_sweep(int op, char transa, char transb, int m, int n, int k, i
     const int lda = nrowa + lda_pad, ldb = nrowb;
     const int esz = op == OP_DGEMM ? 8 : 4, cw = op == OP_CGEMM3M ? 2 : 1;   // reals per element
     int st = 0;
-    void *A = device_workspace(WS_TUNE_A, (size_t)esz * cw * lda * ncola, &st);
-    void *B = st ? nullptr : device_workspace(WS_TUNE_B, (size_t)esz * cw * ldb * ncolb, &st);
-    void *C = st ? nullptr : device_workspace(WS_TUNE_C, (size_t)esz * cw * m * n, &st);
-    void *ws3m = nullptr;
-    if (!st && op == OP_CGEMM3M) ws3m = device_workspace(WS_COMPLEX, gemm3m_workspace_bytes(m, n, k, 4), &st);
-    if (st) return st;
     hipStream_t s = rt().stream;
+    WsLease lease(s);
+    void *A = lease.get(WS_TUNE_A, (size_t)esz * cw * lda * ncola, &st);
+    void *B = st ? nullptr : lease.get(WS_TUNE_B, (size_t)esz * cw * ldb * ncolb, &st);
+    void *C = st ? nullptr : lease.get(WS_TUNE_C, (size_t)esz * cw * m * n, &st);
+    void *ws3m = nullptr;
+    if (!st && op == OP_CGEMM3M) ws3m = lease.get(WS_COMPLEX, gemm3m_workspace_bytes(m, n, k, 4), &st);
+    if (st) return st;
     const int kind = op == OP_DGEMM ? 0 : 1;
     int r = gemm_amd_fill_uniform(A, kind, (int64_t)cw * lda, ncola, (int64_t)cw * lda, 11, 0, 0, s);
     if (!r) r = gemm_amd_fill_uniform(B, kind, (int64_t)cw * ldb, ncolb, (int64_t)cw * ldb, 12, 0, 0, s);
@@ -104,7 +105,7 @@ int gemm_amd_tune_sweep(int op, char transa, char transb, int m, int n, int k, i
         const GemmConfig cfg = i < TUNE_NCFG ? tune_candidate(i) : heur;
         ms[i] = -1.0f;
         if (i < TUNE_NCFG && !tune_valid(op, m, n, k, cfg)) continue;
-        void *work = device_workspace(WS_GEMM, gemm_workspace_bytes(m, n, k, cfg, esz), &st);
+        void *work = lease.get(WS_GEMM, gemm_workspace_bytes(m, n, k, cfg, esz), &st);
         if (st) {
             r = st;
             break;
@@ -196,7 +197,8 @@ int gemm_amd_mlp_forward(const float *params, const float *x, int batch, float *
     if (batch < 0) return -(int)hipErrorInvalidValue;
     if (int st = init_status()) return st;
     int st = 0;
-    float *ws = (float *)device_workspace(WS_MLP, sizeof(float) * mlp_forward_ws_floats(batch) + 256, &st);
+    WsLease lease((hipStream_t)stream);
+    float *ws = (float *)lease.get(WS_MLP, sizeof(float) * mlp_forward_ws_floats(batch) + 256, &st);
     if (st) return st;
     return hip_status((hipError_t)mlp_forward(params, x, batch, y, ws, (hipStream_t)stream), "mlp forward");
 }
@@ -206,7 +208,8 @@ int gemm_amd_mlp_grad(const float *params, const float *x, const float *t, const
     if (batch < 1) return -(int)hipErrorInvalidValue;
     if (int st = init_status()) return st;
     int st = 0;
-    float *ws = (float *)device_workspace(WS_MLP, sizeof(float) * mlp_grad_ws_floats(batch) + 256, &st);
+    WsLease lease((hipStream_t)stream);
+    float *ws = (float *)lease.get(WS_MLP, sizeof(float) * mlp_grad_ws_floats(batch) + 256, &st);
     if (st) return st;
     return hip_status((hipError_t)mlp_grad(params, x, t, mask, batch, grad, loss, ws, (hipStream_t)stream),
                       "mlp grad");
@@ -234,7 +237,8 @@ int gemm_amd_mlp_train(float *params, float *adam_m, float *adam_v, const float 
     const size_t f_x = (size_t)batch * MLP_IN, f_t = (size_t)batch * MLP_OUT;
     const size_t bytes = sizeof(float) * (mlp_grad_ws_floats(batch) + f_x + 2 * f_t + MLP_NPARAMS + nb) +
                          sizeof(int) * (size_t)nsamples + 1024;
-    char *base = (char *)device_workspace(WS_MLP, bytes, &st);
+    WsLease lease(s);
+    char *base = (char *)lease.get(WS_MLP, bytes, &st);
     if (st) return st;
     float *ws = (float *)base;
     float *bx = ws + mlp_grad_ws_floats(batch), *bt = bx + f_x, *bm = bt + f_t, *grad = bm + f_t;

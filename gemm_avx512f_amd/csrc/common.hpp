@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <mutex>
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef double d2_t __attribute__((ext_vector_type(2)));
@@ -86,4 +87,18 @@ __device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, i
     const int r = t - gid * per_group;
     tm = first_m + r % gsize;
     tn = r / gsize;
+}
+
+// Raises the dynamic-LDS limit of the kernel KERN to `bytes`: one
+// hipFuncSetAttribute call per process, made by whichever thread launches the
+// kernel first; every caller gets that call's status.
+template <auto KERN>
+inline hipError_t lds_limit_once(size_t bytes) {
+    static std::once_flag once;
+    static hipError_t status = hipSuccess;
+    std::call_once(once, [bytes] {
+        status = hipFuncSetAttribute(reinterpret_cast<const void *>(KERN), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)bytes);
+    });
+    return status;
 }

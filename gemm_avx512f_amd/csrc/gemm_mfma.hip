@@ -239,13 +239,7 @@ template <typename T, bool TA, bool TB, int BM, int BN, int WM, int WN, int BK, 
 static hipError_t launch_one(const GemmParams<T> &p, int splits, hipStream_t s) {
     constexpr size_t lds = 2ull * SlabLayout<T, TA, TB, BM, BN, BK>::STAGE * sizeof(T);
     auto kern = gemm_mfma_kernel<T, TA, TB, BM, BN, BK, WM, WN, VEC>;
-    static bool attr_set = false;   // per instantiation; set once per process
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
+    if (hipError_t e = lds_limit_once<gemm_mfma_kernel<T, TA, TB, BM, BN, BK, WM, WN, VEC>>(lds)) return e;
     dim3 grid(p.tiles_m * p.tiles_n, splits);
     hipLaunchKernelGGL(kern, grid, dim3(WM * WN * 64), lds, s, p);
     return hipGetLastError();

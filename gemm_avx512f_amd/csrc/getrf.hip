@@ -354,8 +354,12 @@ struct Lu {
 }  // namespace
 
 size_t getrf_gemm_workspace_bytes(int m, int n) {
-    // split-K partials of the widest Schur update: at most 8 partitions of m x n/2
-    return (size_t)8 * (size_t)(m > 0 ? m : 1) * (size_t)((n + 1) / 2 + 1) * sizeof(double);
+    // split-K partials of the widest Schur update, at most 8 partitions of an
+    // m x min(m, n)/2 block (the recursion never updates a wider one; the
+    // solve for the columns right of a wide matrix's square part takes
+    // split-K only where its partials fit, and runs unsplit otherwise)
+    const int mn = m < n ? m : n;
+    return (size_t)8 * (size_t)(m > 0 ? m : 1) * (size_t)((mn + 1) / 2 + 1) * sizeof(double);
 }
 
 int launch_dgetrf(int m, int n, double *A, int64_t lda, int *ipiv, int *info, double *inv_ws, void *gemm_ws,

@@ -362,13 +362,7 @@ static int launch_oz_variant(OzParams &p, hipStream_t s) {
     p.tiles_n = (p.N + BN - 1) / BN;
     constexpr size_t lds = 2ull * (BM + BN) * (KB + 16);
     auto kern = ozaki_gemm_kernel<WM, WN, TM, TN, KB>;
-    static bool attr = false;
-    if (!attr) {
-        hipError_t he =
-            hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (he != hipSuccess) return (int)he;
-        attr = true;
-    }
+    if (hipError_t he = lds_limit_once<ozaki_gemm_kernel<WM, WN, TM, TN, KB>>(lds)) return (int)he;
     hipLaunchKernelGGL(kern, dim3(p.tiles_m * p.tiles_n), dim3(WM * WN * 64), lds, s, p);
     return (int)hipGetLastError();
 }
@@ -952,13 +946,7 @@ static int launch_crt_variant(CrtParams &p, hipStream_t s) {
     p.tiles_n = (p.N + BN - 1) / BN;
     constexpr size_t lds = (size_t)NS * (BM + BN) * (DIRECT ? KB : KB + 16);
     auto kern = ozaki_crt_kernel<WM, WN, TM, TN, KB, DIRECT, NS>;
-    static bool attr = false;
-    if (!attr) {
-        hipError_t he =
-            hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (he != hipSuccess) return (int)he;
-        attr = true;
-    }
+    if (hipError_t he = lds_limit_once<ozaki_crt_kernel<WM, WN, TM, TN, KB, DIRECT, NS>>(lds)) return (int)he;
     hipLaunchKernelGGL(kern, dim3(p.tiles_m * p.tiles_n), dim3(WM * WN * 64), lds, s, p);
     return (int)hipGetLastError();
 }

@@ -4,6 +4,7 @@
 #include "../../include/gemm_amd.h"
 #include <cstdio>
 #include <cstring>
+#include <vector>
 
 namespace gemm_amd {
 
@@ -73,17 +74,94 @@ int DevBuf::ensure(size_t need) {
 static DevBuf g_bufs[BUF_COUNT];
 DevBuf &buf(BufId id) { return g_bufs[id]; }
 
-static std::mutex g_ws_mu;
-static DevBuf g_ws[WS_COUNT];
-void *device_workspace(int slot, size_t bytes, int *status) {
-    std::lock_guard<std::mutex> lk(g_ws_mu);
-    int st = 0;
-    DevBuf &w = g_ws[slot];
-    if (bytes > w.bytes || !w.ptr) {
-        // growth must not free memory another stream may still be using
-        hipError_t e = hipDeviceSynchronize();
-        if (e != hipSuccess) st = hip_status(e, "hipDeviceSynchronize (workspace growth)");
-        if (st == 0) st = w.ensure(bytes);
+struct WsSet {
+    DevBuf ws[WS_COUNT];
+    hipEvent_t done = nullptr;     // recorded on the last holder's stream when it let the set go
+    hipStream_t last = nullptr;    // that stream
+    bool used = false;             // `done` has been recorded
+    bool busy = false;             // held by a lease
+    bool captured = false;         // owned by a captured graph: never reused, never freed
+    std::vector<void *> retired;   // captured sets: buffers outgrown during the capture
+};
+
+static std::mutex g_pool_mu;
+static std::vector<WsSet *> g_pool;   // sets are never destroyed (process lifetime)
+
+static bool stream_capturing(hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive;
+}
+
+WsLease::WsLease(hipStream_t s) : s_(s) {
+    const bool capturing = stream_capturing(s);
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        if (!capturing) {
+            for (WsSet *w : g_pool)   // this stream's own free set first
+                if (!w->busy && !w->captured && w->used && w->last == s) {
+                    set_ = w;
+                    break;
+                }
+            if (!set_)
+                for (WsSet *w : g_pool)
+                    if (!w->busy && !w->captured) {
+                        set_ = w;
+                        break;
+                    }
+        }
+        if (!set_) {
+            set_ = new WsSet();
+            set_->captured = capturing;
+            g_pool.push_back(set_);
+        }
+        set_->busy = true;
+    }
+    if (set_->used && set_->last != s) {
+        // kernels of the previous holder (another stream) may still use the buffers
+        status_ = hip_status(hipStreamWaitEvent(s, set_->done, 0), "hipStreamWaitEvent (workspace hand-over)");
+    }
+}
+
+WsLease::~WsLease() {
+    if (!set_) return;
+    if (!set_->captured) {
+        hipError_t e = hipSuccess;
+        if (!set_->done) e = hipEventCreateWithFlags(&set_->done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(set_->done, s_);
+        if (e == hipSuccess) {
+            set_->last = s_;
+            set_->used = true;
+        } else {
+            // without a recorded event the next holder could not order itself
+            // after this one: drain the stream instead
+            (void)hipStreamSynchronize(s_);
+            set_->used = false;
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    set_->busy = false;
+}
+
+void *WsLease::get(int slot, size_t bytes, int *status) {
+    int st = status_;
+    DevBuf &w = set_->ws[slot];
+    if (st == 0 && (bytes > w.bytes || !w.ptr)) {
+        if (set_->captured) {
+            // the captured kernels keep the old buffer: retire it, never free it
+            hipStreamCaptureMode mode = hipStreamCaptureModeRelaxed;
+            (void)hipThreadExchangeStreamCaptureMode(&mode);
+            if (w.ptr) set_->retired.push_back(w.ptr);
+            w.ptr = nullptr;
+            w.bytes = 0;
+            st = w.ensure(bytes);
+            (void)hipThreadExchangeStreamCaptureMode(&mode);
+        } else {
+            // this stream waits for every earlier holder of the set, and this
+            // lease's own earlier kernels run on it: once it drains, nothing
+            // queued can still use the old buffer
+            if (w.ptr) st = hip_status(hipStreamSynchronize(s_), "hipStreamSynchronize (workspace growth)");
+            if (st == 0) st = w.ensure(bytes);
+        }
     }
     if (status) *status = st;
     return st == 0 ? w.ptr : nullptr;

@@ -70,7 +70,11 @@ void zgemm3m_(const char *transa, const char *transb, const int *m, const int *n
  * argument i, or i > 0 when U(i,i) is exactly zero (the factorisation is still
  * completed).  No counterpart in the reference, which has no LAPACK layer: it
  * is the first caller of the DTRSM and DGEMM paths above, which carry all but
- * the panel columns.  m <= 16384. */
+ * the panel columns.  m <= 16384: a taller matrix gets info = -1 and is left
+ * untouched.  A failure of the GPU runtime (initialisation, memory, a launch
+ * or a copy) gives info = GEMM_AMD_INFO_RUNTIME_ERROR, and a and ipiv are then
+ * not a factorisation. */
+#define GEMM_AMD_INFO_RUNTIME_ERROR (-1000)
 void dgetrf_(const int *m, const int *n, double *a, const int *lda, int *ipiv, int *info);
 
 /* ----------------------------------------------------- Part 1b: CBLAS and ILP64

@@ -95,3 +95,24 @@ def test_python_binding_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
     with pytest.raises(_lib.GemmAmdError):
         _lib.lib()
+
+
+def test_dgetrf_host_rejects_too_tall_matrix(capfd):
+    # m above the supported maximum: info = -1 before any device work, the
+    # matrix and the pivots untouched (LAPACK callers read info != 0 as failure)
+    from gemm_avx512f_amd import blas
+    m, n = 16385, 2
+    a = np.asfortranarray(np.full((m, n), 3.0))
+    ipiv = np.full(n, -7, np.int32)
+    info = blas.dgetrf_(m, n, a, m, ipiv)
+    assert info == -1
+    assert np.all(a == 3.0) and np.all(ipiv == -7)
+    assert "DGETRF" in capfd.readouterr().err
+    # the ILP64 symbol passes the same info through and leaves its pivots alone
+    L = _lib.lib()
+    m64, n64, lda64, info64 = (ctypes.c_int64(v) for v in (m, n, m, 0))
+    piv64 = np.full(n, -7, np.int64)
+    L.dgetrf_64_(ctypes.byref(m64), ctypes.byref(n64), a.ctypes.data_as(ctypes.c_void_p), ctypes.byref(lda64),
+                 piv64.ctypes.data_as(ctypes.c_void_p), ctypes.byref(info64))
+    assert info64.value == -1
+    assert np.all(a == 3.0) and np.all(piv64 == -7)

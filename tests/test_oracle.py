@@ -324,3 +324,23 @@ def test_ozaki_crt_split_residues_rebuild_the_scaled_integers():
     rc, exc = O.ozaki_crt_split(x, 7, 40, by="cols")
     rr, exr = O.ozaki_crt_split(np.asfortranarray(x.T), 7, 40, by="rows")
     assert np.array_equal(rc, rr) and np.array_equal(exc, exr)
+
+
+def test_err_metrics_known_answers():
+    # hand-computed: c - r = (0, -0.5, 1); max|r| = 2.5; ||c - r|| = sqrt(1.25), ||r|| = sqrt(11.25)
+    c = np.array([1.0, 2.0, 3.0])
+    r = np.array([1.0, 2.5, 2.0])
+    e = O.err(c, r)
+    assert e["max_rel"] == pytest.approx(0.4, rel=1e-15)
+    assert e["frob_rel"] == pytest.approx(1.0 / 3.0, rel=1e-15)
+    assert e["max_elem_rel"] == pytest.approx(0.5, rel=1e-15)
+    e32 = O.err(c.astype(np.float32), r)
+    assert (e32["max_rel"], e32["frob_rel"], e32["max_elem_rel"]) == pytest.approx((0.4, 1.0 / 3.0, 0.5), rel=1e-15)
+    # the low word of a double-double reference is added, not subtracted or dropped
+    hi = np.array([1.0, 2.0, 4.0])
+    lo = np.array([2.0 ** -60, 0.0, -(2.0 ** -55)])
+    exact = O.err(hi, hi, lo)
+    assert exact["max_rel"] == pytest.approx(2.0 ** -55 / 4.0, rel=1e-12)
+    assert O.err(hi + 0.0, hi, None)["max_rel"] == 0.0
+    # an all-zero reference reports the absolute error
+    assert O.err(np.array([0.0, 0.25]), np.zeros(2))["max_rel"] == 0.25
