@@ -54,6 +54,8 @@ SIGNATURES = {
     # device-pointer variants
     "gemm_amd_dtrsm": (c_int, [c_char, c_char, c_char, c_char, c_int, c_int, c_double, c_vp, c_int, c_vp, c_int,
                                c_vp]),
+    "gemm_amd_dtrsm_noalloc": (c_int, [c_char, c_char, c_char, c_char, c_int, c_int, c_double, c_vp, c_int, c_vp, c_int,
+                               c_vp]),
     "gemm_amd_dtrsm_kernel": (c_int, [c_int, c_long, c_long, c_vp, c_vp, c_vp, c_long, c_long, c_vp]),
     "gemm_amd_dgetrf": (c_int, [c_int, c_int, c_vp, c_int, c_vp, c_vp, c_vp]),
     "gemm_amd_sgemm": (c_int, [c_char, c_char, c_int, c_int, c_int, c_float, c_vp, c_int, c_vp, c_int,

@@ -8,10 +8,10 @@
 
 using namespace gemm_amd;
 
-namespace {
+namespace gemm_amd {
 
 // netlib dtrsm argument order: side, uplo, transa, diag, m, n, alpha, a, lda, b, ldb
-int trsm_check(char side, char uplo, char transa, char diag, int m, int n, int lda, int ldb) {
+int trsm_check_args(char side, char uplo, char transa, char diag, int m, int n, int lda, int ldb) {
     const bool left = lsame(side, 'L');
     const int nrowa = left ? m : n;
     if (!left && !lsame(side, 'R')) return 1;
@@ -24,6 +24,10 @@ int trsm_check(char side, char uplo, char transa, char diag, int m, int n, int l
     if (ldb < (m > 1 ? m : 1)) return 11;
     return 0;
 }
+
+}  // namespace gemm_amd
+
+namespace {
 
 int trsm_run(char side, char uplo, char transa, char diag, int m, int n, double alpha, const double *a, int lda,
              double *b, int ldb, hipStream_t s) {
@@ -48,7 +52,7 @@ extern "C" {
 
 int gemm_amd_dtrsm(char side, char uplo, char transa, char diag, int m, int n, double alpha, const double *a,
                    int lda, double *b, int ldb, void *stream) {
-    const int info = trsm_check(side, uplo, transa, diag, m, n, lda, ldb);
+    const int info = trsm_check_args(side, uplo, transa, diag, m, n, lda, ldb);
     if (info) {
         xerbla("DTRSM", info);
         return info;
@@ -61,7 +65,7 @@ int gemm_amd_dtrsm(char side, char uplo, char transa, char diag, int m, int n, d
 void dtrsm_(const char *side, const char *uplo, const char *transa, const char *diag, const int *pm, const int *pn,
             const double *palpha, const double *a, const int *plda, double *b, const int *pldb) {
     const int m = *pm, n = *pn, lda = *plda, ldb = *pldb;
-    const int info = trsm_check(*side, *uplo, *transa, *diag, m, n, lda, ldb);
+    const int info = trsm_check_args(*side, *uplo, *transa, *diag, m, n, lda, ldb);
     if (info) {
         xerbla("DTRSM", info);
         return;

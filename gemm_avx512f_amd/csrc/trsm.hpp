@@ -6,6 +6,9 @@ namespace gemm_amd {
 
 size_t trsm_workspace_inv_bytes(int na);
 
+// netlib DTRSM argument check: 0, or the index of the first illegal argument.
+int trsm_check_args(char side, char uplo, char transa, char diag, int m, int n, int lda, int ldb);
+
 // Blocked recursive DTRSM on device pointers (op(A) = A or A^T, triangle =
 // upper/lower as stored).  inv_ws: trsm_workspace_inv_bytes; gemm_ws may be
 // null (no split-K).

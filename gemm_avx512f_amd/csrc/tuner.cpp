@@ -278,6 +278,11 @@ void table_clear() {
     P.table.clear();
 }
 
+static thread_local bool t_pick_without_device = false;
+
+PickWithoutDevice::PickWithoutDevice() : prev_(t_pick_without_device) { t_pick_without_device = true; }
+PickWithoutDevice::~PickWithoutDevice() { t_pick_without_device = prev_; }
+
 GemmConfig pick_gemm_config(int dtype, bool ta, bool tb, int m, int n, int k, int lda_pad, int cus) {
     static const char *ft = getenv("GEMM_AMD_FORCE_TILE");
     static const char *fs = getenv("GEMM_AMD_FORCE_SPLITK");
@@ -300,6 +305,12 @@ GemmConfig pick_gemm_config(int dtype, bool ta, bool tb, int m, int n, int k, in
     const Key key = make_key(op, ta, tb, m, n, k, lda_pad);
     auto it = P.table.find(key);
     if (it != P.table.end()) return it->second;
+    if (t_pick_without_device) {
+        // only what is already known: the picker's cached choice, else the heuristic
+        if (!P.defaults_tried || P.mode != PICK_MLP || !P.loaded) return c;
+        auto ct = P.cache.find(key);
+        return ct != P.cache.end() ? ct->second : c;
+    }
     decide_defaults_locked(P);
     if (P.mode != PICK_MLP || !P.loaded) return c;
     auto ct = P.cache.find(key);

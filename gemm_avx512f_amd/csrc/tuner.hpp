@@ -32,6 +32,21 @@ bool tune_valid(int op, int m, int n, int k, const GemmConfig &c);
 enum PickMode : int { PICK_HEURISTIC = 0, PICK_MLP = 1 };
 
 GemmConfig pick_gemm_config(int dtype, bool ta, bool tb, int m, int n, int k, int lda_pad, int cus);
+
+// While one lives on a thread, pick_gemm_config on that thread does no device
+// work: a shape that neither the sweep table nor the picker's cache holds gets
+// the heuristic's config instead of a network inference (whose first use
+// allocates the picker's device buffers).  For callers that must not allocate.
+class PickWithoutDevice {
+public:
+    PickWithoutDevice();
+    ~PickWithoutDevice();
+    PickWithoutDevice(const PickWithoutDevice &) = delete;
+    PickWithoutDevice &operator=(const PickWithoutDevice &) = delete;
+
+private:
+    bool prev_;
+};
 GemmConfig heuristic_gemm_config(int dtype, int m, int n, int k, int cus);
 
 // Learned picker control (thread-safe).

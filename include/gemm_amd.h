@@ -125,6 +125,20 @@ int gemm_amd_zgemm3m(char transa, char transb, int m, int n, int k, const double
 int gemm_amd_dtrsm(char side, char uplo, char transa, char diag, int m, int n, double alpha,
                    const double *a, int lda, double *b, int ldb, void *stream);
 
+/* gemm_amd_dtrsm that allocates nothing: no counterpart in the reference, for
+ * callers that cannot allocate device memory (code under graph capture, tight
+ * serving loops, harnesses that forbid it).  Same arguments, result and
+ * kernels as gemm_amd_dtrsm; its scratch (inverted diagonal blocks, split-K
+ * partials of the trailing updates) is a fixed 64 MiB device array per GPU,
+ * held until the process ends.  The triangle's order (m for side 'L', n for
+ * 'R') must be at most GEMM_AMD_DTRSM_NOALLOC_MAX_NA, otherwise nothing runs
+ * and -hipErrorInvalidValue (-1) is returned with a message.  Calls share
+ * their GPU's array: calls that overlap in time on one GPU race and must be
+ * ordered by the caller. */
+#define GEMM_AMD_DTRSM_NOALLOC_MAX_NA 32768
+int gemm_amd_dtrsm_noalloc(char side, char uplo, char transa, char diag, int m, int n, double alpha,
+                           const double *a, int lda, double *b, int ldb, void *stream);
+
 /* Kernel-level solve with the reference's packed-operand contract, on device
  * pointers: replaces dtrsm_solve_{ln,lt,rn,rt}_m{M}n{N}(sa, sb, C, LDB, LDC)
  * (dtrsm_kernel_16x4_skylakex.S:1882-2536; semantics as its tester's

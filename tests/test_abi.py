@@ -116,3 +116,12 @@ def test_dgetrf_host_rejects_too_tall_matrix(capfd):
                  piv64.ctypes.data_as(ctypes.c_void_p), ctypes.byref(info64))
     assert info64.value == -1
     assert np.all(a == 3.0) and np.all(piv64 == -7)
+
+
+def test_noalloc_dtrsm_rejects_before_device_work():
+    # argument checks and the scratch limit come before any device work
+    L = _lib.lib()
+    assert L.gemm_amd_dtrsm_noalloc(b"X", b"L", b"N", b"N", 4, 4, 1.0, None, 4, None, 4, None) == 1
+    assert L.gemm_amd_dtrsm_noalloc(b"R", b"L", b"N", b"N", 4, 32769, 1.0, None, 32769, None, 4, None) == -1
+    assert "exceeds the maximum of 32768" in _lib.last_error()
+    assert L.gemm_amd_dtrsm_noalloc(b"L", b"L", b"N", b"N", 0, 4, 1.0, None, 1, None, 1, None) == 0
