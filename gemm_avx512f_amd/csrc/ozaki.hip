@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <mutex>
+#include <type_traits>
 
 namespace gemm_amd {
 
@@ -466,7 +467,7 @@ static constexpr int OZ_CRT_CHUNK_K = 131008;   // 131008 * 128^2 < 2^31
 
 // grid (Kp/64, Rp/64): residues [l][Rp][Kp] (zero padding) and exps[r] = e_r - al.
 __global__ __launch_bounds__(256) void oz_residue_kernel(const double *__restrict__ X, int64_t rs, int64_t ks, int R,
-                                                         int K, int Rp, int Kp, int al,
+                                                         int K, int Rp, int Kp, int pitch, int al,
                                                          const unsigned long long *__restrict__ rowmax,
                                                          const CrtConst *__restrict__ cc, int8_t *__restrict__ out,
                                                          int *__restrict__ exps) {
@@ -491,8 +492,8 @@ __global__ __launch_bounds__(256) void oz_residue_kernel(const double *__restric
         xh[i] = floor(x * 0x1p-32);
         xl[i] = fma(-xh[i], 0x1p32, x);
     }
-    const int64_t plane = (int64_t)Rp * Kp;
-    int8_t *dst = out + (int64_t)(r0 + rr) * Kp + k0 + kq;
+    const int64_t plane = (int64_t)Rp * pitch;
+    int8_t *dst = out + (int64_t)(r0 + rr) * pitch + k0 + kq;
     const int nmod = cc->nmod;
     for (int l = 0; l < nmod; ++l) {
         const double m = (double)cc->m[l], im = cc->invm[l], c = cc->c32[l];
@@ -509,6 +510,7 @@ __global__ __launch_bounds__(256) void oz_residue_kernel(const double *__restric
 
 struct CrtParams {
     int M, N, Kp, tiles_m, tiles_n;
+    int pitch;                    // bytes between rows of a residue plane (>= Kp)
     int64_t planeA, planeB;       // bytes per residue plane
     const int8_t *RA, *RB;
     const int *ea, *eb;
@@ -541,10 +543,12 @@ typedef __attribute__((address_space(1))) void global_void_t;
 // 128-byte rows, (r >> 2) & 3 for 64-byte rows: the 16-byte fragment reads of
 // 16 consecutive rows fall on 16 distinct 16-byte groups of the 256-byte bank
 // width.
-__device__ __forceinline__ void wait_vmem_at_most(int n) {   // n wave-uniform
-    if (n >= 16) asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+__device__ __forceinline__ void wait_vmem_at_most(int n) {   // n wave-uniform; waits for at most n (rounded down)
+    if (n >= 32) asm volatile("s_waitcnt vmcnt(32)" ::: "memory");
+    else if (n >= 16) asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
     else if (n >= 8) asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
     else if (n >= 4) asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+    else if (n >= 2) asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
     else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
@@ -579,17 +583,17 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
     i4_t ra[PA], rb[PB];
     auto gload = [&](int st) {
         const int l = st / nkt, kt = st - l * nkt;
-        const int8_t *a = p.RA + (int64_t)l * p.planeA + (int64_t)m0 * p.Kp + kt * KB;
-        const int8_t *b = p.RB + (int64_t)l * p.planeB + (int64_t)n0 * p.Kp + kt * KB;
+        const int8_t *a = p.RA + (int64_t)l * p.planeA + (int64_t)m0 * p.pitch + kt * KB;
+        const int8_t *b = p.RB + (int64_t)l * p.planeB + (int64_t)n0 * p.pitch + kt * KB;
 #pragma unroll
         for (int i = 0; i < PA; ++i) {
             const int v = tid + i * NT, row = v / CH, ch = (v % CH) * 16;
-            ra[i] = *reinterpret_cast<const i4_t *>(a + (int64_t)row * p.Kp + ch);
+            ra[i] = *reinterpret_cast<const i4_t *>(a + (int64_t)row * p.pitch + ch);
         }
 #pragma unroll
         for (int i = 0; i < PB; ++i) {
             const int v = tid + i * NT, row = v / CH, ch = (v % CH) * 16;
-            rb[i] = *reinterpret_cast<const i4_t *>(b + (int64_t)row * p.Kp + ch);
+            rb[i] = *reinterpret_cast<const i4_t *>(b + (int64_t)row * p.pitch + ch);
         }
     };
     auto sstore = [&](unsigned char *buf) {
@@ -619,13 +623,13 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
     // even wave count its swizzle, and so its byte offset in the row, does not
     // depend on j: one per-lane offset plus a uniform base per instruction
     const int lrow = lane / CPR;
-    const int loff = lrow * p.Kp + 16 * ((lane % CPR) ^ swz(RPI * wave + lrow));
+    const int loff = lrow * p.pitch + 16 * ((lane % CPR) ^ swz(RPI * wave + lrow));
     auto dload = [&](int st) {
         const int l = st / nkt, kt = st - l * nkt;
         unsigned char *buf = smem + (st % NS) * STAGE;
-        const int8_t *ua = p.RA + (int64_t)l * p.planeA + (int64_t)(m0 + RPI * wave) * p.Kp + kt * KB;
-        const int8_t *ub = p.RB + (int64_t)l * p.planeB + (int64_t)(n0 + RPI * wave) * p.Kp + kt * KB;
-        const int64_t js = (int64_t)(RPI * NW) * p.Kp;
+        const int8_t *ua = p.RA + (int64_t)l * p.planeA + (int64_t)(m0 + RPI * wave) * p.pitch + kt * KB;
+        const int8_t *ub = p.RB + (int64_t)l * p.planeB + (int64_t)(n0 + RPI * wave) * p.pitch + kt * KB;
+        const int64_t js = (int64_t)(RPI * NW) * p.pitch;
 #pragma unroll
         for (int j = 0; j < LPW; ++j) {
             const int8_t *u = j < JA ? ua + j * js : ub + (j - JA) * js;
@@ -835,6 +839,287 @@ __global__ __launch_bounds__(WM *WN * 64) void ozaki_crt_kernel(CrtParams p) {
     }
 }
 
+// Wide form of the CRT product kernel: one 256x256 output tile per workgroup
+// of 4 waves, one wave per SIMD, each wave a 128x128 block of 4x4
+// v_mfma_i32_32x32x32_i8 tiles (256 int32 accumulators per lane, held in the
+// accumulation registers).  Against the 8-wave 128x64 form it reads 8
+// fragments per 16 matrix instructions instead of 6 per 8, so LDS carries a
+// third less per operation, and one wave per SIMD leaves the other half of
+// the register file to the fragments and addresses.
+//
+// Pipeline (NS stages of KB k-bytes each, rows XOR-swizzled as above):
+//   * stage st is multiplied out of LDS buffer st % NS while the direct
+//     loads of stages st+1 .. st+NS-1 are in flight;
+//   * a k-step's fragments are read while the previous k-step's matrix
+//     instructions run (read and matrix instructions interleaved);
+//   * in the last k-step of stage st, after its first matrix instructions:
+//     wait for this wave's loads of stage st+1, one barrier (all waves have
+//     then landed stage st+1 and finished reading stage st), issue the loads
+//     of stage st+NS into buffer st % NS and read stage st+1's first
+//     fragments, all while the rest of the k-step's matrix instructions run.
+//     The barrier is the stage's only synchronisation point.
+//   * when a modulus' last stage is done, its residues leave the
+//     accumulators (exact FP64 quotient; m = 256 is the low byte) as one
+//     16-byte store per 32x32 tile, four residues per word.
+// K must fit one exact int32 chunk (Kp <= OZ_CRT_CHUNK_K); the caller uses
+// ozaki_crt_kernel otherwise.
+template <int KB, int NS, int PARTS = 1>
+__global__ __launch_bounds__(256, 1) void ozaki_crt_wide_kernel(CrtParams p) {
+    static_assert(KB == 128 || KB == 64, "64- or 128-byte rows");
+    static_assert(PARTS == 1 || (NS == 2 && PARTS <= KB / 32), "split issue: two stages, a part per k-step");
+    constexpr int NW = 4, BM = 256, BN = 256;
+    constexpr int CPR = KB / 16;                  // 16-byte chunks per row
+    constexpr int RPI = 64 / CPR;                 // rows per load instruction
+    constexpr int STAGE = (BM + BN) * KB;         // bytes per stage
+    constexpr int LPW = (BM + BN) / RPI / NW;     // load instructions per wave per stage
+    constexpr int JA = BM / (RPI * NW);           // of which fill A rows
+    constexpr int KS = KB / 32;                   // k-steps per stage
+    static_assert(KS % 2 == 0, "the next stage's first fragments go to fragment set 0");
+    extern __shared__ __align__(16) unsigned char smem[];
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm0 = (wave & 1) * 128, wn0 = (wave >> 1) * 128;
+    int tm, tn;
+    tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, 8, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int nkt = p.Kp / KB;
+    const int nmod = p.cc->nmod;
+    const int total = nmod * nkt;
+    const int64_t wv = (int64_t)(tm * p.tiles_n + tn) * NW + wave;
+    // residues of this wave: [modulus][tile ab][lane] x 16 bytes
+    i4_t *Rv = reinterpret_cast<i4_t *>(p.Rv) + wv * ((int64_t)nmod * 16 * 64) + lane;
+
+    auto swz = [](int row) { return CPR == 8 ? (row >> 1) & 7 : (row >> 2) & 3; };
+    // direct loads: instruction j of wave w fills stage rows RPI (w + NW j) ..
+    // + RPI - 1 (A rows first, then B rows); lane l loads chunk
+    // (l % CPR) ^ sw(row) of row RPI (w + NW j) + l / CPR, whose swizzle does
+    // not depend on j, so one per-lane offset serves every instruction
+    const int lrow = lane / CPR;
+    const int loff = lrow * p.pitch + 16 * ((lane % CPR) ^ swz(RPI * wave + lrow));
+    const int64_t jstride = (int64_t)(RPI * NW) * p.pitch;
+    // stage sd's loads (sd < total), one instruction at a time: dma_setup
+    // fixes the stage's addresses, dma_one(j) issues instruction j; dl, dkt:
+    // the next stage's modulus and k-tile
+    int dl = 0, dkt = 0;
+    unsigned char *dbuf = nullptr;
+    const int8_t *dua = nullptr, *dub = nullptr;
+    auto dma_setup = [&](int sd) {
+        dbuf = smem + (sd % NS) * STAGE + RPI * wave * KB;
+        dua = p.RA + (int64_t)dl * p.planeA + (int64_t)(m0 + RPI * wave) * p.pitch + dkt * KB + loff;
+        dub = p.RB + (int64_t)dl * p.planeB + (int64_t)(n0 + RPI * wave) * p.pitch + dkt * KB + loff;
+        if (++dkt == nkt) {
+            dkt = 0;
+            ++dl;
+        }
+    };
+    auto dma_one = [&](int j) {
+        const int8_t *u = j < JA ? dua + j * jstride : dub + (j - JA) * jstride;
+        __builtin_amdgcn_global_load_lds((global_void_t *)u, (lds_void_t *)(dbuf + RPI * NW * j * KB), 16, 0, 0);
+    };
+
+    // fragment reads: rows wm0 + 32 s + q (A) and 256 + wn0 + 32 s + q (B),
+    // chunk 2 ks + g of the lane's k-step, swizzled by the row (32-row blocks
+    // and the 256-row B offset keep the swizzle of q)
+    const int q = lane & 31, g = lane >> 5, qx = swz(q);
+    int ko[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) ko[ks] = ((2 * ks + g) ^ qx) << 4;
+    const int aoff = (wm0 + q) * KB, boff = (BM + wn0 + q) * KB;
+    i4_t fa[2][4], fb[2][4];
+    // fragment f of k-step ks (f < 4: A block f, else B block f - 4)
+    auto read_one = [&](const unsigned char *stg, int ks, int f, i4_t *A4, i4_t *B4) {
+        if (f < 4) A4[f] = *reinterpret_cast<const i4_t *>(stg + aoff + 32 * f * KB + ko[ks]);
+        else B4[f - 4] = *reinterpret_cast<const i4_t *>(stg + boff + 32 * (f - 4) * KB + ko[ks]);
+    };
+    i16_t acc[4][4];
+    auto mfma = [&](int i, const i4_t *A4, const i4_t *B4) {
+        acc[i >> 2][i & 3] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A4[i >> 2], B4[i & 3], acc[i >> 2][i & 3], 0, 0, 0);
+    };
+
+    if (total > 0) {
+        const int pre = min(NS, total);
+        for (int i = 0; i < pre; ++i) {
+            dma_setup(i);
+#pragma unroll
+            for (int j = 0; j < LPW; ++j) dma_one(j);
+        }
+        wait_vmem_at_most(LPW * (pre - 1));   // this wave's part of stage 0 has landed
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int f = 0; f < 8; ++f) read_one(smem, 0, f, fa[0], fb[0]);
+    }
+    // One k-step: its 16 matrix instructions, each followed by at most one
+    // fragment read for the next k-step and at most one direct load, in that
+    // order (the scheduler may not move anything across the groups): the
+    // reads and loads issue while the matrix pipe is busy instead of holding
+    // it up in a block.
+    int st = 0;
+    for (int l = 0; l < nmod; ++l) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[a][b][e] = 0;
+        for (int kt = 0; kt < nkt; ++kt, ++st) {
+            const unsigned char *cur = smem + (st % NS) * STAGE;
+            // stage st+NS's loads are issued in PARTS parts behind matrix
+            // instructions: part 0 behind this stage's last k-step, part p
+            // behind k-step p-1 of the next stage
+            constexpr int PART = LPW / PARTS;
+            const bool later_parts = PARTS > 1 && st >= 1 && st + 1 < total;
+#pragma unroll
+            for (int ks = 0; ks < KS - 1; ++ks) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    mfma(i, fa[ks & 1], fb[ks & 1]);
+                    if (i < 8) read_one(cur, ks + 1, i, fa[(ks + 1) & 1], fb[(ks + 1) & 1]);
+                    if (PARTS > 1 && ks < PARTS - 1 && i < PART && later_parts) dma_one((ks + 1) * PART + i);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            // last k-step: the stage hand-over runs behind its matrix instructions
+            constexpr int L = KS - 1;
+            mfma(0, fa[L & 1], fb[L & 1]);
+            mfma(1, fa[L & 1], fb[L & 1]);
+            __builtin_amdgcn_sched_barrier(0);
+            if (st + 1 < total) {
+                // stages st+2 .. min(st+NS-1, total-1) may stay in flight
+                const int later = min(st + NS - 1, total - 1) - (st + 1);
+                wait_vmem_at_most(LPW * later);
+            }
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
+            const bool issue = st + NS < total, next = st + 1 < total;
+            if (issue) dma_setup(st + NS);
+            const unsigned char *nxt = smem + ((st + 1) % NS) * STAGE;
+            constexpr int NLAST = PART;   // loads behind this k-step
+#pragma unroll
+            for (int i = 2; i < 16; ++i) {
+                mfma(i, fa[L & 1], fb[L & 1]);
+                if (i - 2 < 8 && next) read_one(nxt, 0, i - 2, fa[0], fb[0]);
+                if constexpr (NLAST <= 14) {
+                    if (i - 2 < NLAST && issue) dma_one(i - 2);
+                } else {
+                    // more loads than matrix instructions: two per instruction
+                    if (2 * (i - 2) < NLAST && issue) dma_one(2 * (i - 2));
+                    if (2 * (i - 2) + 1 < NLAST && issue) dma_one(2 * (i - 2) + 1);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        // residues of modulus l straight from the accumulators
+        const int m = p.cc->m[l];
+        const double md = (double)m, im = p.cc->invm[l];
+#pragma unroll
+        for (int ab = 0; ab < 16; ++ab) {
+            const i16_t v = acc[ab >> 2][ab & 3];
+            i4_t w;
+#pragma unroll
+            for (int e4 = 0; e4 < 4; ++e4) {
+                unsigned word = 0;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int x = v[4 * e4 + e];
+                    int r = x;   // m = 256: the low byte is the residue
+                    if (m != 256) {
+                        const double xd = (double)x;
+                        r = (int)__builtin_fma(-__builtin_rint(xd * im), md, xd);   // exact: |x| < 2^31
+                    }
+                    word |= (unsigned)(r & 0xFF) << (8 * e);
+                }
+                w[e4] = (int)word;
+            }
+            Rv[((int64_t)l * 16 + ab) * 64] = w;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// CRT reconstruction after ozaki_crt_wide_kernel, one thread per (wave,
+// 32x32 tile, lane) of the product kernel: the thread loads that lane's
+// residues of the tile under every modulus at once (16 bytes per modulus,
+// 1 KB per modulus across a wave) and rebuilds its 16 elements of C.  Run as
+// its own launch it has the whole chip and many waves in flight for what is
+// a memory-bound pass; inside the product kernel it ran with one wave per
+// SIMD and nothing to hide its loads behind.
+__global__ __launch_bounds__(256) void ozaki_crt_rebuild_kernel(CrtParams p) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int lane = (int)(t & 63), ab = (int)((t >> 6) & 15);
+    const int64_t wv = t >> 10;
+    const int wave = (int)(wv & 3), tile = (int)(wv >> 2);
+    const int tm = tile / p.tiles_n, tn = tile - tm * p.tiles_n;
+    const int m0 = tm * 256 + (wave & 1) * 128, n0 = tn * 256 + (wave >> 1) * 128;
+    const int a = ab >> 2, b = ab & 3, q = lane & 31, g = lane >> 5;
+    const int nmod = p.cc->nmod;
+    const i4_t *Rv = reinterpret_cast<const i4_t *>(p.Rv) + wv * ((int64_t)nmod * 16 * 64) + ab * 64 + lane;
+    i4_t R[OZ_CRT_MAXMOD];
+#pragma unroll
+    for (int l = 0; l < OZ_CRT_MAXMOD; ++l) R[l] = l < nmod ? Rv[(int64_t)l * 16 * 64] : i4_t{0, 0, 0, 0};
+    const int j = n0 + 32 * b + q;
+    if (j >= p.N) return;
+    const int eb = p.eb[j];
+    const double two32 = 4294967296.0;
+#pragma unroll
+    for (int e4 = 0; e4 < 4; ++e4) {
+        double s[4][4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) s[e][k] = 0.0;
+        // moduli past nmod have zero residues and zero weights
+#pragma unroll
+        for (int l = 0; l < OZ_CRT_MAXMOD; ++l) {
+            const unsigned word = (unsigned)R[l][e4];
+            const double w0 = p.cc->w[l][0], w1 = p.cc->w[l][1], w2 = p.cc->w[l][2], w3 = p.cc->w[l][3];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const double r = (double)(int)(signed char)(word >> (8 * e));
+                s[e][0] = fma(r, w0, s[e][0]);
+                s[e][1] = fma(r, w1, s[e][1]);
+                s[e][2] = fma(r, w2, s[e][2]);
+                s[e][3] = fma(r, w3, s[e][3]);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int ee = 4 * e4 + e;
+            const int i = m0 + 32 * a + (ee >> 2) * 8 + g * 4 + (ee & 3);
+            if (i >= p.M) continue;
+            const double approx = (s[e][3] * two32 + s[e][2]) * (two32 * two32) + s[e][1] * two32;
+            const double qq = rint(approx * p.cc->invM);
+            long long t0 = (long long)fma(-qq, p.cc->Mk[0], s[e][0]);
+            long long t1 = (long long)fma(-qq, p.cc->Mk[1], s[e][1]);
+            long long t2 = (long long)fma(-qq, p.cc->Mk[2], s[e][2]);
+            long long t3 = (long long)fma(-qq, p.cc->Mk[3], s[e][3]);
+            // carries: t0, t1, t2 into [0, 2^32) (floor division by 2^32)
+            constexpr long long B32 = 4294967296ll;
+            long long c = t0 >> 32;
+            t0 -= c * B32;
+            t1 += c;
+            c = t1 >> 32;
+            t1 -= c * B32;
+            t2 += c;
+            c = t2 >> 32;
+            t2 -= c * B32;
+            t3 += c;
+            // V = hi 2^64 + lo with lo taken signed, so small |V| loses nothing
+            long long hi = t3 * B32 + t2;
+            const unsigned long long ulo = ((unsigned long long)t1 << 32) | (unsigned long long)t0;
+            const long long lo = (long long)ulo;
+            if (ulo >> 63) hi += 1;
+            const double V = fma((double)hi, two32 * two32, (double)lo);
+            const double v = p.alpha * ldexp(V, p.ea[i] + eb);
+            double *dst = p.C + i + (int64_t)j * p.ldc;
+            *dst = (p.beta == 0.0) ? v : v + p.beta * *dst;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- host side of the CRT form
 static const int kCrtModuli[OZ_CRT_MAXMOD] = {256, 255, 253, 251, 247, 241, 239, 233,
                                               229, 227, 223, 217, 211, 199, 197, 193};
@@ -908,8 +1193,13 @@ static const CrtConst *crt_device_constants(int nmod, int *status) {
     return table[nmod];
 }
 
+// Row pitch of the residue planes: k padded to 128 bytes, plus 128 bytes when
+// that is a multiple of 512, so that the rows a stage loads (one pitch apart)
+// do not all start at the same offset modulo a large power of two.
+static int crt_pitch(int Kp) { return Kp % 512 == 0 ? Kp + 128 : Kp; }
+
 size_t ozaki_crt_workspace_bytes(int rows, int k, int nmod) {
-    const size_t rp = (size_t)ozaki_pad_rows(rows), kp = (size_t)ozaki_pad_k(k);
+    const size_t rp = (size_t)ozaki_pad_rows(rows), kp = (size_t)crt_pitch(ozaki_pad_k(k));
     return (size_t)nmod * rp * kp + 256 + rp * sizeof(unsigned long long) + rp * sizeof(int) + 256;
 }
 
@@ -920,10 +1210,10 @@ size_t ozaki_crt_accum_bytes(int m, int n, int nmod) {
 
 static int crt_split(const double *X, int64_t rs, int64_t ks, int R, int K, int al, const CrtConst *cc, int nmod,
                      void *ws, int8_t **res, int **exps, hipStream_t s) {
-    const int Rp = ozaki_pad_rows(R), Kp = ozaki_pad_k(K);
+    const int Rp = ozaki_pad_rows(R), Kp = ozaki_pad_k(K), pitch = crt_pitch(Kp);
     unsigned char *base = static_cast<unsigned char *>(ws);
     *res = reinterpret_cast<int8_t *>(base);
-    size_t off = ((size_t)nmod * Rp * Kp + 255) & ~(size_t)255;
+    size_t off = ((size_t)nmod * Rp * pitch + 255) & ~(size_t)255;
     unsigned long long *rowmax = reinterpret_cast<unsigned long long *>(base + off);
     off += (size_t)Rp * sizeof(unsigned long long);
     *exps = reinterpret_cast<int *>(base + off);
@@ -931,8 +1221,8 @@ static int crt_split(const double *X, int64_t rs, int64_t ks, int R, int K, int 
     if (K > 0 && R > 0)
         hipLaunchKernelGGL(oz_rowmax_kernel, dim3((K + 63) / 64, (R + 63) / 64), dim3(256), 0, s, X, rs, ks, R, K,
                            rowmax);
-    hipLaunchKernelGGL(oz_residue_kernel, dim3(Kp / 64, Rp / 64), dim3(256), 0, s, X, rs, ks, R, K, Rp, Kp, al,
-                       rowmax, cc, *res, *exps);
+    hipLaunchKernelGGL(oz_residue_kernel, dim3(Kp / 64, Rp / 64), dim3(256), 0, s, X, rs, ks, R, K, Rp, Kp, pitch,
+                       al, rowmax, cc, *res, *exps);
     return (int)hipGetLastError();
 }
 
@@ -948,6 +1238,21 @@ static int launch_crt_variant(CrtParams &p, hipStream_t s) {
     auto kern = ozaki_crt_kernel<WM, WN, TM, TN, KB, DIRECT, NS>;
     if (hipError_t he = lds_limit_once<ozaki_crt_kernel<WM, WN, TM, TN, KB, DIRECT, NS>>(lds)) return (int)he;
     hipLaunchKernelGGL(kern, dim3(p.tiles_m * p.tiles_n), dim3(WM * WN * 64), lds, s, p);
+    return (int)hipGetLastError();
+}
+
+template <int KB, int NS, int PARTS = 1>
+static int launch_crt_wide(CrtParams &p, hipStream_t s) {
+    if (p.Kp > OZ_CRT_CHUNK_K) return launch_crt_variant<2, 4, 4, 2, 128, true>(p, s);   // K over one exact chunk
+    p.tiles_m = (p.M + 255) / 256;
+    p.tiles_n = (p.N + 255) / 256;
+    constexpr size_t lds = (size_t)NS * 512 * KB;
+    auto kern = ozaki_crt_wide_kernel<KB, NS, PARTS>;
+    if (hipError_t he = lds_limit_once<ozaki_crt_wide_kernel<KB, NS, PARTS>>(lds)) return (int)he;
+    hipLaunchKernelGGL(kern, dim3(p.tiles_m * p.tiles_n), dim3(256), lds, s, p);
+    if (hipError_t he = hipGetLastError()) return (int)he;
+    // 4 waves x 16 tiles x 64 lanes per 256x256 tile, 256 threads per block
+    hipLaunchKernelGGL(ozaki_crt_rebuild_kernel, dim3(p.tiles_m * p.tiles_n * 16), dim3(256), 0, s, p);
     return (int)hipGetLastError();
 }
 
@@ -969,8 +1274,9 @@ int launch_dgemm_ozaki_crt(bool ta, bool tb, int m, int n, int k, double alpha, 
     p.M = m;
     p.N = n;
     p.Kp = ozaki_pad_k(k);
-    p.planeA = (int64_t)ozaki_pad_rows(m) * p.Kp;
-    p.planeB = (int64_t)ozaki_pad_rows(n) * p.Kp;
+    p.pitch = crt_pitch(p.Kp);
+    p.planeA = (int64_t)ozaki_pad_rows(m) * p.pitch;
+    p.planeB = (int64_t)ozaki_pad_rows(n) * p.pitch;
     p.RA = ra;
     p.RB = rb;
     p.ea = ea;
@@ -986,13 +1292,18 @@ int launch_dgemm_ozaki_crt(bool ta, bool tb, int m, int n, int k, double alpha, 
     p.ldc = ldc;
     static const char *var = getenv("GEMM_AMD_CRT_VARIANT");
     const int v = var ? atoi(var) : 0;
-    // 256x256 tiles of 8 waves (128x64 each), 128-deep stages; the default
-    // stages through global_load_lds (measured 84.9 against 81.4 TFLOP/s
-    // FP64-equivalent for register staging at 8192^3, 16 moduli)
-    if (v == 1) return launch_crt_variant<2, 4, 4, 2, 128>(p, s);   // register staging
-    if (v == 2) return launch_crt_variant<2, 4, 4, 2, 64>(p, s);    // register staging, 64-deep stages
-    if (v == 4) return launch_crt_variant<2, 4, 4, 2, 64, true, 4>(p, s);   // direct, 4 stages of 64 bytes
-    return launch_crt_variant<2, 4, 4, 2, 128, true>(p, s);
+    // Default: 256x256 tiles of 4 waves (128x128 each), two 128-byte direct
+    // stages, each stage's loads issued in two halves behind matrix
+    // instructions, reconstruction in its own launch (at 8192^3, 16 moduli,
+    // whole call: 113.9 TFLOP/s FP64-equivalent against 103.4 for the 8-wave
+    // form).  The others stay selectable for comparison (same results):
+    if (v == 1) return launch_crt_variant<2, 4, 4, 2, 128>(p, s);   // 8 waves, register staging
+    if (v == 2) return launch_crt_variant<2, 4, 4, 2, 64>(p, s);    // 8 waves, register staging, 64-deep stages
+    if (v == 3) return launch_crt_variant<2, 4, 4, 2, 128, true>(p, s);   // 8 waves, direct, 2 stages
+    if (v == 4) return launch_crt_variant<2, 4, 4, 2, 64, true, 4>(p, s);   // 8 waves, direct, 4 stages of 64 bytes
+    if (v == 5) return launch_crt_wide<128, 2>(p, s);       // 4 waves, loads issued in one block
+    if (v == 6) return launch_crt_wide<64, 4>(p, s);        // 4 waves, four 64-byte stages
+    return launch_crt_wide<128, 2, 2>(p, s);
 }
 
 int launch_ozaki_crt_split_only(const double *X, int64_t rs, int64_t ks, int R, int K, int nmod, int al,
@@ -1004,7 +1315,7 @@ int launch_ozaki_crt_split_only(const double *X, int64_t rs, int64_t ks, int R, 
     hipLaunchKernelGGL(oz_zero_u64_kernel, dim3(grid1(Rp)), dim3(256), 0, s, rowmax_ws, Rp);
     hipLaunchKernelGGL(oz_rowmax_kernel, dim3((K + 63) / 64, (R + 63) / 64), dim3(256), 0, s, X, rs, ks, R, K,
                        rowmax_ws);
-    hipLaunchKernelGGL(oz_residue_kernel, dim3(Kp / 64, Rp / 64), dim3(256), 0, s, X, rs, ks, R, K, Rp, Kp, al,
+    hipLaunchKernelGGL(oz_residue_kernel, dim3(Kp / 64, Rp / 64), dim3(256), 0, s, X, rs, ks, R, K, Rp, Kp, Kp, al,
                        rowmax_ws, cc, res, exps);
     return (int)hipGetLastError();
 }
