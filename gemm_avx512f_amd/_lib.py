@@ -72,6 +72,8 @@ SIGNATURES = {
                                      c_vp, c_vp]),
     "gemm_amd_dgemm_ozaki_crt": (c_int, [c_char, c_char, c_int, c_int, c_int, c_double, c_vp, c_int, c_vp, c_int,
                                          c_double, c_vp, c_int, c_int, c_vp]),
+    "gemm_amd_emulation_counts": (c_int, [c_int, c_vp, c_vp]),
+    "gemm_amd_emulation_counts_set": (c_int, [c_vp, c_vp, c_int]),
     "gemm_amd_ozaki_crt_bits": (c_int, [c_int, c_int]),
     "gemm_amd_ozaki_crt_split": (c_int, [c_vp, ctypes.c_int64, ctypes.c_int64, c_int, c_int, c_int, c_int, c_vp,
                                          c_int, c_int, c_vp, c_vp]),
@@ -81,6 +83,8 @@ SIGNATURES = {
                                     c_vp]),
     "gemm_amd_grouped_create": (c_vp, [c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                        c_vp, c_vp, c_vp]),
+    "gemm_amd_grouped_create_ex": (c_vp, [c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                       c_vp, c_vp, c_vp, c_int]),
     "gemm_amd_grouped_run": (c_int, [c_vp, c_vp]),
     "gemm_amd_grouped_capture": (c_int, [c_vp, c_vp]),
     "gemm_amd_grouped_replay": (c_int, [c_vp, c_vp]),

@@ -5,8 +5,69 @@
 #include "bf16x.hpp"
 #include "gemm_kernels.hpp"
 #include "../../include/gemm_amd.h"
+#include <mutex>
 
 using namespace gemm_amd;
+
+namespace {
+
+#include "emulation_counts.inc"
+
+// Counts chosen per ceil(log2 K): the compiled-in autotuner table until a
+// caller installs another (gemm_amd_emulation_counts_set).
+std::mutex g_counts_mu;
+int g_moduli[33], g_slices[33];
+bool g_counts_init = false;
+
+int log2_bucket(int k) {
+    int b = 0;
+    while (b < 32 && (1ll << b) < (long long)k) ++b;
+    return b;
+}
+
+void counts_for(int k, int *nmod, int *nsl) {
+    std::lock_guard<std::mutex> lk(g_counts_mu);
+    if (!g_counts_init) {
+        for (int i = 0; i < 33; ++i) {
+            g_moduli[i] = kCrtAutoModuli[i];
+            g_slices[i] = kSliceAutoCount[i];
+        }
+        g_counts_init = true;
+    }
+    const int b = log2_bucket(k);
+    if (nmod) *nmod = g_moduli[b];
+    if (nsl) *nsl = g_slices[b];
+}
+
+}  // namespace
+
+extern "C" int gemm_amd_emulation_counts(int k, int *nmoduli, int *nslices) {
+    if (k < 1) {
+        set_error("gemm_amd_emulation_counts: k >= 1");
+        return -(int)hipErrorInvalidValue;
+    }
+    counts_for(k, nmoduli, nslices);
+    return 0;
+}
+
+extern "C" int gemm_amd_emulation_counts_set(const int *moduli, const int *slices, int n) {
+    if (n != 33 || !moduli || !slices) {
+        set_error("gemm_amd_emulation_counts_set: 33 entries (ceil(log2 K) = 0..32) of each");
+        return -(int)hipErrorInvalidValue;
+    }
+    for (int i = 0; i < 33; ++i)
+        if (moduli[i] < 2 || moduli[i] > OZ_CRT_MAXMOD || slices[i] < 1 || slices[i] > 8) {
+            set_error("gemm_amd_emulation_counts_set: moduli 2..16 and slices 1..8");
+            return -(int)hipErrorInvalidValue;
+        }
+    counts_for(1, nullptr, nullptr);   // initialise, then replace
+    std::lock_guard<std::mutex> lk(g_counts_mu);
+    for (int i = 0; i < 33; ++i) {
+        g_moduli[i] = moduli[i];
+        g_slices[i] = slices[i];
+    }
+    return 0;
+}
 
 extern "C" int gemm_amd_sgemm_bf16x(char ta, char tb, int m, int n, int k, float alpha, const float *a, int lda,
                                     const float *b, int ldb, float beta, float *c, int ldc, int nprod, void *stream) {
@@ -58,10 +119,11 @@ extern "C" int gemm_amd_dgemm_ozaki(char ta, char tb, int m, int n, int k, doubl
         xerbla("DGEMM_OZAKI", info);
         return info;
     }
-    if (nslices < 1 || nslices > 8) {
-        set_error("gemm_amd_dgemm_ozaki: nslices must be 1..8");
+    if (nslices < 0 || nslices > 8) {
+        set_error("gemm_amd_dgemm_ozaki: nslices must be 1..8, or 0 for the tuned count for this K");
         return -(int)hipErrorInvalidValue;
     }
+    if (nslices == 0) counts_for(k > 0 ? k : 1, nullptr, &nslices);
     if (m == 0 || n == 0 || ((alpha == 0.0 || k == 0) && beta == 1.0)) return 0;
     if (int st = init_status()) return st;
     hipStream_t s = (hipStream_t)stream;
@@ -106,10 +168,11 @@ extern "C" int gemm_amd_dgemm_ozaki_crt(char ta, char tb, int m, int n, int k, d
         xerbla("DGEMM_OZAKI_CRT", info);
         return info;
     }
-    if (nmoduli < 2 || nmoduli > OZ_CRT_MAXMOD) {
-        set_error("gemm_amd_dgemm_ozaki_crt: nmoduli must be 2..16");
+    if (nmoduli != 0 && (nmoduli < 2 || nmoduli > OZ_CRT_MAXMOD)) {
+        set_error("gemm_amd_dgemm_ozaki_crt: nmoduli must be 2..16, or 0 for the tuned count for this K");
         return -(int)hipErrorInvalidValue;
     }
+    if (nmoduli == 0) counts_for(k > 0 ? k : 1, &nmoduli, nullptr);
     if (m == 0 || n == 0 || ((alpha == 0.0 || k == 0) && beta == 1.0)) return 0;
     if (ozaki_crt_bits(nmoduli, k) < 1) {
         set_error("gemm_amd_dgemm_ozaki_crt: too few moduli for this K (no bits left per operand)");

@@ -32,21 +32,29 @@ int check_all(const char *name, int count, const char *ta, const char *tb, const
 
 extern "C" {
 
-void *gemm_amd_grouped_create(int dtype, int count, const char *transa, const char *transb, const int *m,
-                              const int *n, const int *k, const double *alpha, const void *const *a, const int *lda,
-                              const void *const *b, const int *ldb, const double *beta, void *const *c,
-                              const int *ldc) {
-    if (dtype < 0 || dtype > 2 || count < 0) {
-        set_error("gemm_amd_grouped_create: dtype must be 0 (FP32), 1 (BF16 in, FP32 out) or 2 (FP64); count >= 0");
+void *gemm_amd_grouped_create_ex(int dtype, int count, const char *transa, const char *transb, const int *m,
+                                 const int *n, const int *k, const double *alpha, const void *const *a,
+                                 const int *lda, const void *const *b, const int *ldb, const double *beta,
+                                 void *const *c, const int *ldc, int large_min) {
+    if (dtype < 0 || dtype > 2 || count < 0 || large_min < 0) {
+        set_error("gemm_amd_grouped_create: dtype must be 0 (FP32), 1 (BF16 in, FP32 out) or 2 (FP64); count >= 0; "
+                  "large_min >= 0");
         return nullptr;
     }
     if (check_all("GROUPED_GEMM", count, transa, transb, m, n, k, lda, ldb, ldc)) return nullptr;
     if (init_status()) return nullptr;
     int st = 0;
     void *plan = grouped_plan_create(dtype, count, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
-                                     rt().cus, &st);
+                                     rt().cus, large_min, &st);
     if (!plan) hip_status((hipError_t)st, "grouped plan");
     return plan;
+}
+
+void *gemm_amd_grouped_create(int dtype, int count, const char *transa, const char *transb, const int *m,
+                              const int *n, const int *k, const double *alpha, const void *const *a, const int *lda,
+                              const void *const *b, const int *ldb, const double *beta, void *const *c,
+                              const int *ldc) {
+    return gemm_amd_grouped_create_ex(dtype, count, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, 0);
 }
 
 int gemm_amd_grouped_run(void *plan, void *stream) {
@@ -141,7 +149,7 @@ int gemm_amd_zgemm3m_batched(int count, const char *transa, const char *transb, 
     }
     void *plan = grouped_plan_create(GROUPED_F64, 3 * count, ta3.data(), tb3.data(), m3.data(), n3.data(), k3.data(),
                                      al3.data(), A3.data(), la3.data(), B3.data(), lb3.data(), be3.data(), C3.data(),
-                                     lc3.data(), rt().cus, &st);
+                                     lc3.data(), rt().cus, 0, &st);
     if (!plan) return hip_status((hipError_t)st, "ZGEMM3M_BATCHED plan");
     // the host tables stay alive until the synchronisation below
     int e = (int)hipMemcpyAsync(d_split, sj.data(), tabs, hipMemcpyHostToDevice, s);

@@ -25,6 +25,7 @@ namespace gemm_amd {
 struct GroupDesc {
     int m, n, k, ta, tb, tiles_m, tiles_n, vec;   // vec: A, B 16-byte aligned with ld a multiple of 16 bytes
     int chunked;                                  // vec, and m, n, k multiples of the 16-byte chunk
+    int cls;                                      // tile class: 0 = small tiles, 1 = large tiles
     int64_t lda, ldb, ldc, tile_base;
     const void *A, *B;
     void *C;
@@ -188,9 +189,14 @@ __device__ __forceinline__ void grouped_dispatch(const GroupDesc &d, int tm, int
     }
 }
 
-template <typename TI, typename TO, int BM, int BN, int BK, int WM, int WN>
-__global__ __launch_bounds__(WM *WN * 64) void grouped_gemm_kernel(const GroupDesc *__restrict__ descs, int count,
-                                                                   int64_t total_tiles) {
+// Two tile classes in one launch (4 waves each): small (S) for problems that
+// a large tile would mostly pad, large (L) for the rest; the host picks the
+// class per problem.  A plan whose problems are all small launches the
+// instantiation with L = S, whose registers (the small class alone) keep
+// more waves resident.
+template <typename TI, typename TO, typename S, typename L>
+__global__ __launch_bounds__(256) void grouped_gemm_kernel(const GroupDesc *__restrict__ descs, int count,
+                                                           int64_t total_tiles) {
     extern __shared__ __align__(16) unsigned char smem[];
     for (int64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
         int lo = 0, hi = count - 1;
@@ -207,7 +213,12 @@ __global__ __launch_bounds__(WM *WN * 64) void grouped_gemm_kernel(const GroupDe
         // instead of being hoisted out of this loop and held live in registers
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        grouped_dispatch<TI, TO, BM, BN, BK, WM, WN>(d, tm, tn, smem, tid);
+        if constexpr (std::is_same_v<S, L>) {
+            grouped_dispatch<TI, TO, S::BM, S::BN, S::BK, S::WM, S::WN>(d, tm, tn, smem, tid);
+        } else {
+            if (d.cls) grouped_dispatch<TI, TO, L::BM, L::BN, L::BK, L::WM, L::WN>(d, tm, tn, smem, tid);
+            else grouped_dispatch<TI, TO, S::BM, S::BN, S::BK, S::WM, S::WN>(d, tm, tn, smem, tid);
+        }
         lds_barrier();   // the next tile reuses the LDS stages
     }
 }
@@ -220,51 +231,73 @@ struct GroupedPlan {
     int grid = 0;
     size_t lds = 0;
     const void *kernel = nullptr;
+    int64_t resident = 0;   // workgroups resident at once
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
 
-// Tile configuration per dtype: 64x64 tiles of four 32x32 waves.  The grouped
-// problems (tens to a few thousand per dimension) leave large tiles short of
-// work: with the register-staged loads of the first version, 128x128 tiles
-// measured 26 / 63 TFLOP/s (FP32 / bf16) on 512 problems of 32..2048 against
-// 44 / 117 for these 64x64 ones; with the chunked two-ahead pipeline, bf16
-// problems of M, N >= 128 on 128x128 tiles (32-deep slabs) measured 111 against
-// 128 TFLOP/s on the same mix (their transposing LDS stores conflict more).
-// bf16 takes 64-deep slabs (77 TFLOP/s with 32-deep ones in the first
-// version) so a stage holds more matrix work per barrier.  FP32 and FP64 use
-// the single-problem kernel's 64x64 tile and slab depth, so the k order (and
-// the result) is the same bit for bit.
+// Tile classes per dtype, four waves each.  Small: 64x64 tiles of 32x32
+// waves, the single-problem kernel's 64x64 tile and slab depth (FP32, FP64),
+// so the k order (and the result) is the same bit for bit.  Large: the
+// single-problem kernel's main tiles with the same slab depths (FP32 128x128
+// of 64x64 waves, FP64 128x64 of 64x32 waves; bf16 128x128 of 64x64 waves,
+// 64-deep slabs): four times (twice) the matrix work per fragment read and
+// per barrier.  A problem takes the large class when both of its output
+// dimensions reach the plan's threshold (FP64: M that, N half of it);
+// grouped_default_large_min gives the default per dtype.
+struct TileCfg64x64F32 { static constexpr int BM = 64, BN = 64, BK = 32, WM = 2, WN = 2; };
+struct TileCfg128F32 { static constexpr int BM = 128, BN = 128, BK = 32, WM = 2, WN = 2; };
+struct TileCfg64x64F64 { static constexpr int BM = 64, BN = 64, BK = 16, WM = 2, WN = 2; };
+struct TileCfg128x64F64 { static constexpr int BM = 128, BN = 64, BK = 16, WM = 2, WN = 2; };
+struct TileCfg64x64BF16 { static constexpr int BM = 64, BN = 64, BK = 64, WM = 2, WN = 2; };
+struct TileCfg128BF16 { static constexpr int BM = 128, BN = 128, BK = 64, WM = 2, WN = 2; };
 template <typename TI, typename TO>
 struct GroupedCfg;
-template <> struct GroupedCfg<float, float> { static constexpr int BM = 64, BN = 64, BK = 32, WM = 2, WN = 2; };
-template <> struct GroupedCfg<double, double> { static constexpr int BM = 64, BN = 64, BK = 16, WM = 2, WN = 2; };
-template <> struct GroupedCfg<uint16_t, float> { static constexpr int BM = 64, BN = 64, BK = 64, WM = 2, WN = 2; };
+template <> struct GroupedCfg<float, float> { using S = TileCfg64x64F32; using L = TileCfg128F32; };
+template <> struct GroupedCfg<double, double> { using S = TileCfg64x64F64; using L = TileCfg128x64F64; };
+template <> struct GroupedCfg<uint16_t, float> { using S = TileCfg64x64BF16; using L = TileCfg128BF16; };
 
+template <typename TI, typename C>
+constexpr size_t stage_bytes() {
+    return 2ull * (C::BM + C::BN) * (C::BK + 16 / sizeof(TI)) * sizeof(TI);
+}
+
+// LDS for the classes the plan uses (a plan with small tiles only keeps more
+// workgroups resident), and the resident-grid size at that LDS.
 template <typename TI, typename TO>
-static int prepare(GroupedPlan &P) {
-    using C = GroupedCfg<TI, TO>;
-    constexpr int PITCH = C::BK + 16 / (int)sizeof(TI);
-    P.lds = 2ull * (C::BM + C::BN) * PITCH * sizeof(TI);
-    auto kern = grouped_gemm_kernel<TI, TO, C::BM, C::BN, C::BK, C::WM, C::WN>;
-    P.kernel = reinterpret_cast<const void *>(kern);
-    hipError_t e = hipFuncSetAttribute(P.kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds);
+static int prepare(GroupedPlan &P, int cus, bool any_large) {
+    using G = GroupedCfg<TI, TO>;
+    const size_t ls = stage_bytes<TI, typename G::S>(), ll = stage_bytes<TI, typename G::L>();
+    P.lds = any_large && ll > ls ? ll : ls;
+    const void *kern = any_large
+                           ? reinterpret_cast<const void *>(grouped_gemm_kernel<TI, TO, typename G::S, typename G::L>)
+                           : reinterpret_cast<const void *>(grouped_gemm_kernel<TI, TO, typename G::S, typename G::S>);
+    P.kernel = kern;
+    // the limit covers either class, so plans of both kinds can be launched in any order
+    hipError_t e = hipFuncSetAttribute(P.kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(ll > ls ? ll : ls));
+    // persistent grid: as many workgroups as can be resident at once
+    int per_cu = 0;
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, P.lds);
+    P.resident = (int64_t)cus * (per_cu > 0 ? per_cu : 1);
     return (int)e;
 }
 
-static int tile_bm_of(int dtype) {
-    return dtype == GROUPED_F64 ? GroupedCfg<double, double>::BM
-                                : (dtype == GROUPED_F32 ? GroupedCfg<float, float>::BM : GroupedCfg<uint16_t, float>::BM);
+template <typename TI, typename TO>
+static void tile_dims(int cls, int *bm, int *bn) {
+    using G = GroupedCfg<TI, TO>;
+    *bm = cls ? G::L::BM : G::S::BM;
+    *bn = cls ? G::L::BN : G::S::BN;
 }
-static int tile_bn_of(int dtype) {
-    return dtype == GROUPED_F64 ? GroupedCfg<double, double>::BN
-                                : (dtype == GROUPED_F32 ? GroupedCfg<float, float>::BN : GroupedCfg<uint16_t, float>::BN);
+static void tile_dims_of(int dtype, int cls, int *bm, int *bn) {
+    if (dtype == GROUPED_F64) tile_dims<double, double>(cls, bm, bn);
+    else if (dtype == GROUPED_F32) tile_dims<float, float>(cls, bm, bn);
+    else tile_dims<uint16_t, float>(cls, bm, bn);
 }
 
 void *grouped_plan_create(int dtype, int count, const char *ta, const char *tb, const int *m, const int *n,
                           const int *k, const double *alpha, const void *const *A, const int *lda,
                           const void *const *B, const int *ldb, const double *beta, void *const *Cp, const int *ldc,
-                          int cus, int *status) {
+                          int cus, int large_min, int *status) {
     *status = 0;
     GroupedPlan *P = new GroupedPlan();
     P->dtype = dtype;
@@ -277,7 +310,8 @@ void *grouped_plan_create(int dtype, int count, const char *ta, const char *tb, 
     });
     std::vector<GroupDesc> h(count);
     int64_t base = 0;
-    const int BM = tile_bm_of(dtype), BN = tile_bn_of(dtype);
+    bool any_large = false;
+    if (large_min <= 0) large_min = grouped_default_large_min(dtype);
     for (int i = 0; i < count; ++i) {
         const int p = order[i];
         GroupDesc &d = h[i];
@@ -286,6 +320,10 @@ void *grouped_plan_create(int dtype, int count, const char *ta, const char *tb, 
         d.k = k[p];
         d.ta = !(ta[p] == 'N' || ta[p] == 'n');
         d.tb = !(tb[p] == 'N' || tb[p] == 'n');
+        d.cls = m[p] >= large_min && n[p] >= (dtype == GROUPED_F64 ? large_min / 2 : large_min);
+        any_large = any_large || d.cls;
+        int BM, BN;
+        tile_dims_of(dtype, d.cls, &BM, &BN);
         d.tiles_m = (m[p] + BM - 1) / BM;
         d.tiles_n = (n[p] + BN - 1) / BN;
         {
@@ -317,12 +355,11 @@ void *grouped_plan_create(int dtype, int count, const char *ta, const char *tb, 
         e = hipMemcpy(P->d_descs, h.data(), sizeof(GroupDesc) * count, hipMemcpyHostToDevice);
     int st = (int)e;
     if (!st) {
-        if (dtype == GROUPED_F32) st = prepare<float, float>(*P);
-        else if (dtype == GROUPED_F64) st = prepare<double, double>(*P);
-        else st = prepare<uint16_t, float>(*P);
+        if (dtype == GROUPED_F32) st = prepare<float, float>(*P, cus, any_large);
+        else if (dtype == GROUPED_F64) st = prepare<double, double>(*P, cus, any_large);
+        else st = prepare<uint16_t, float>(*P, cus, any_large);
     }
-    // persistent grid: four resident workgroups per CU
-    int64_t g = (int64_t)cus * 4;
+    int64_t g = P->resident;
     if (g > P->total_tiles) g = P->total_tiles;
     P->grid = (int)(g > 0 ? g : 1);
     if (st) {

@@ -30,9 +30,10 @@ class GroupedGemm:
     k, alpha, a, b, beta, c (a, b, c are DeviceMatrix; their ``ld`` is used).
 
     dtype: GROUPED_F32, GROUPED_BF16 (operands hold bf16 bit patterns as
-    uint16, C is FP32) or GROUPED_F64."""
+    uint16, C is FP32) or GROUPED_F64.  large_min: smallest M and N of a
+    problem on the large tile class (None: the library's default, 128)."""
 
-    def __init__(self, dtype, problems):
+    def __init__(self, dtype, problems, large_min=None):
         self.dtype = dtype
         self.problems = list(problems)
         for p in self.problems:
@@ -51,7 +52,10 @@ class GroupedGemm:
             _arr(ctypes.c_double, [p["beta"] for p in P]),
             _arr(ctypes.c_void_p, [p["c"].ptr for p in P]), _arr(ctypes.c_int, [p["c"].ld for p in P]),
         ]
-        self.handle = L.gemm_amd_grouped_create(dtype, cnt, *self._keep)
+        if large_min is None:
+            self.handle = L.gemm_amd_grouped_create(dtype, cnt, *self._keep)
+        else:
+            self.handle = L.gemm_amd_grouped_create_ex(dtype, cnt, *self._keep, int(large_min))
         if not self.handle:
             raise _lib.GemmAmdError("grouped_create: " + _lib.last_error())
 

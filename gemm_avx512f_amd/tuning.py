@@ -8,11 +8,15 @@
   log2(time / best time).  It is trained on the GPU by the library's own HIP
   kernels (:func:`train`), and run by its HIP inference kernel
   (:func:`picker_choose`, and inside every GEMM call for unseen shapes).
+* :func:`tune_emulation` picks the moduli / slice counts of the emulated
+  DGEMM per K (the counts the library uses when called with 0), and
+  :func:`tune_grouped` the tile-class threshold of a grouped workload.
 
 Command line (runs on a GPU host)::
 
     python -m gemm_avx512f_amd.tuning sweep --seed 1 --start 0 --count 1000 --out sweep.npz
     python -m gemm_avx512f_amd.tuning train --data a.npz b.npz --out picker.npz
+    python -m gemm_avx512f_amd.tuning tune-emulation --out counts.json --write-inc
 """
 import argparse
 import ctypes
@@ -250,6 +254,153 @@ def write_weights_inc(params, path=WEIGHTS_INC, note=""):
         f.write("\n".join(lines))
 
 
+# ------------------------------------------------------------------ emulation counts
+COUNTS_INC = os.path.join(HERE, "csrc", "emulation_counts.inc")
+CRT_COUNTS = (16, 15, 14, 13, 12, 11, 10, 9, 8)
+SLICE_COUNTS = (8, 7, 6, 5, 4, 3)
+
+
+def _max_rel(c, ref):
+    return float(np.max(np.abs(c - ref)) / np.max(np.abs(ref)))
+
+
+def tune_emulation(ks=tuple(2 ** e for e in range(6, 18)), ranges=(0, 8, 32), size=512, seed=31, log=None):
+    """Per K, the smallest CRT moduli count and slice count whose max relative
+    error stays at or below the native FP64 kernel's for A, B entries
+    u * 2^e, e uniform in [-r, r], at every r.  Errors are taken against the
+    16-moduli CRT result, whose own error is far below both (about 2^-55
+    relative at K = 8192 against native's 2^-49), so the comparison needs no
+    host reference; every product runs on the GPU through the library.
+    Returns {k: {"moduli": n, "slices": s, "errors": {...}}}."""
+    from . import blas
+    from .device import FILL_F64_RANGE, DeviceMatrix, fill_uniform, synchronize
+
+    out = {}
+    for k in ks:
+        A, B = DeviceMatrix(size, k, np.float64), DeviceMatrix(k, size, np.float64)
+        C = DeviceMatrix(size, size, np.float64)
+        ok_crt = {n: True for n in CRT_COUNTS}
+        ok_sl = {n: True for n in SLICE_COUNTS}
+        errs = {}
+        for r in ranges:
+            d = fill_uniform(A, seed + r, 0, kind=FILL_F64_RANGE, range_r=r)
+            fill_uniform(B, seed + r, d, kind=FILL_F64_RANGE, range_r=r)
+            blas.dgemm_ozaki_crt("N", "N", size, size, k, 1.0, A, B, 0.0, C, nmoduli=16)
+            synchronize()
+            ref = C.to_host()
+            blas.dgemm("N", "N", size, size, k, 1.0, A, B, 0.0, C)
+            synchronize()
+            nat = _max_rel(C.to_host(), ref)
+            e = {"native": nat}
+            for n in CRT_COUNTS[1:]:
+                if _lib.lib().gemm_amd_ozaki_crt_bits(n, k) < 1:
+                    ok_crt[n] = False
+                    continue
+                blas.dgemm_ozaki_crt("N", "N", size, size, k, 1.0, A, B, 0.0, C, nmoduli=n)
+                synchronize()
+                e["crt%d" % n] = _max_rel(C.to_host(), ref)
+                ok_crt[n] = ok_crt[n] and e["crt%d" % n] <= nat
+            for n in SLICE_COUNTS:
+                blas.dgemm_ozaki("N", "N", size, size, k, 1.0, A, B, 0.0, C, nslices=n)
+                synchronize()
+                e["slices%d" % n] = _max_rel(C.to_host(), ref)
+                ok_sl[n] = ok_sl[n] and e["slices%d" % n] <= nat
+            errs["r=%d" % r] = e
+        # the smallest count that passes, with every larger count passing too
+        nm = 16
+        for n in CRT_COUNTS[1:]:
+            if not ok_crt[n]:
+                break
+            nm = n
+        ns = 8
+        for n in SLICE_COUNTS:
+            if not ok_sl[n]:
+                break
+            ns = n
+        if not ok_sl[8]:
+            ns = 8   # nothing fewer than the most accurate count
+        out[int(k)] = {"moduli": nm, "slices": ns, "errors": errs}
+        if log:
+            log("K %d: %d moduli, %d slices" % (k, nm, ns))
+        del A, B, C
+    return out
+
+
+def counts_tables(tuned):
+    """33-entry tables (index ceil(log2 K)) from tune_emulation's result: each
+    bucket takes the count of the nearest tuned K at or above it (the counts
+    grow with K), the largest tuned K's beyond the range."""
+    ks = sorted(tuned)
+    mod, sl = [], []
+    for b in range(33):
+        kk = 1 << b
+        sel = [k for k in ks if k >= kk]
+        src = tuned[sel[0]] if sel else tuned[ks[-1]]
+        if not sel:   # beyond the tuned range: the most accurate counts
+            src = {"moduli": 16, "slices": 8}
+        mod.append(int(src["moduli"]))
+        sl.append(int(src["slices"]))
+    return mod, sl
+
+
+def write_counts_inc(tuned, path=COUNTS_INC, note=""):
+    mod, sl = counts_tables(tuned)
+    fmt = lambda v: "\n".join("    " + ", ".join(str(x) for x in v[i:i + 16]) + "," for i in range(0, 33, 16))  # noqa: E731
+    text = ("// Moduli (CRT form) and slices (slice form) of the emulated DGEMM per\n"
+            "// ceil(log2 K), the smallest counts whose max error stayed at or below the\n"
+            "// native FP64 kernel's on inputs of dynamic range 2^0, 2^8 and 2^32, written\n"
+            "// by gemm_avx512f_amd/tuning.py (tune-emulation)." + (" " + note if note else "") + "\n"
+            "static const int kCrtAutoModuli[33] = {\n" + fmt(mod) + "\n};\n"
+            "static const int kSliceAutoCount[33] = {\n" + fmt(sl) + "\n};\n")
+    with open(path, "w") as f:
+        f.write(text)
+
+
+def emulation_counts(k):
+    """The (moduli, slices) the library uses for this K when called with 0."""
+    nm, ns = ctypes.c_int(), ctypes.c_int()
+    _lib.check(_lib.lib().gemm_amd_emulation_counts(k, ctypes.byref(nm), ctypes.byref(ns)), "emulation_counts")
+    return nm.value, ns.value
+
+
+def emulation_counts_set(tuned):
+    mod, sl = counts_tables(tuned)
+    a, b = (ctypes.c_int * 33)(*mod), (ctypes.c_int * 33)(*sl)
+    _lib.check(_lib.lib().gemm_amd_emulation_counts_set(a, b, 33), "emulation_counts_set")
+
+
+# ------------------------------------------------------------------ grouped tile classes
+GROUPED_THRESHOLDS = (64, 96, 128, 192, 256, 1 << 30)
+
+
+def tune_grouped(dtype, problems, thresholds=GROUPED_THRESHOLDS, replays=10, stream=None):
+    """Time one grouped workload (GroupedGemm problems) for each tile-class
+    threshold (smallest M and N of a problem on the large tiles; 2^30 = small
+    tiles only) as a captured graph's replay; returns (best threshold,
+    {threshold: ms per replay})."""
+    from .device import Event, Stream
+    from .grouped import GroupedGemm
+
+    stream = stream or Stream()
+    times = {}
+    for t in thresholds:
+        g = GroupedGemm(dtype, problems, large_min=t)
+        try:
+            g.capture(stream)
+            for _ in range(2):
+                g.replay(stream)
+            stream.synchronize()
+            e0, e1 = Event(), Event()
+            e0.record(stream)
+            for _ in range(replays):
+                g.replay(stream)
+            e1.record(stream)
+            times[int(t)] = e0.elapsed_ms(e1) / replays
+        finally:
+            g.destroy()
+    return min(times, key=times.get), times
+
+
 def regret_report(shapes, ms, choice):
     """Median / mean / 90th-percentile regret of `choice` and of the heuristic, overall and per op."""
     r_pick = regret(ms, np.asarray(choice))
@@ -307,6 +458,9 @@ def _main(argv):
     ev.add_argument("--weights", required=True)
     ev.add_argument("--data", nargs="+", required=True)
     ev.add_argument("--out", default=None)
+    te = sub.add_parser("tune-emulation", help="moduli / slice counts of the emulated DGEMM per K")
+    te.add_argument("--out", required=True)
+    te.add_argument("--write-inc", action="store_true", help="also rewrite csrc/emulation_counts.inc")
     one = sub.add_parser("shape", help="time every candidate of one shape and print TFLOP/s per candidate")
     one.add_argument("--op", type=int, required=True)
     one.add_argument("--ta", type=int, default=0)
@@ -317,6 +471,16 @@ def _main(argv):
     one.add_argument("--pad", type=int, default=0)
     a = ap.parse_args(argv)
     log = lambda msg: print(msg, flush=True)  # noqa: E731
+    if a.cmd == "tune-emulation":
+        import json
+        _lib.check(_lib.lib().gemm_amd_init(), "init")
+        tuned = tune_emulation(log=log)
+        with open(a.out, "w") as f:
+            json.dump({str(k): v for k, v in tuned.items()}, f, indent=1)
+        if a.write_inc:
+            write_counts_inc(tuned)
+        log("wrote " + a.out)
+        return 0
     if a.cmd == "shape":
         ms = sweep(a.op, bool(a.ta), bool(a.tb), a.m, a.n, a.k, a.pad)
         fl = 2.0 * a.m * a.n * a.k * (3 if a.op == OP_CGEMM3M else 1)

@@ -179,7 +179,8 @@ int gemm_amd_dgemm_cfg(char transa, char transb, int m, int n, int k, double alp
  * with exact int32 accumulation, and FP64 recombines them (Ozaki-Ogita-Oishi
  * splitting; int8 form after Ootomo-Ozaki-Yokota 2024).  Accuracy ~2^(-7*nslices)
  * relative to the row/column maxima; nslices = 8 matches or beats the native
- * FP64 kernel's error on uniform inputs. */
+ * FP64 kernel's error on uniform inputs.  nslices = 0 takes the count the
+ * emulation autotuner found for this K (gemm_amd_emulation_counts). */
 int gemm_amd_dgemm_ozaki(char transa, char transb, int m, int n, int k, double alpha,
                          const double *a, int lda, const double *b, int ldb, double beta,
                          double *c, int ldc, int nslices, void *stream);
@@ -201,12 +202,23 @@ int gemm_amd_ozaki_split(const double *x, int64_t row_stride, int64_t k_stride, 
  * 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193), multiplied as
  * one exact int8 GEMM per modulus, and rebuilt by the Chinese remainder
  * theorem in exact FP64/int64 limb arithmetic.  nmoduli int8 GEMMs in all;
- * 16 keeps 55 bits per operand at K = 8192. */
+ * 16 keeps 55 bits per operand at K = 8192.  nmoduli = 0 takes the count the
+ * emulation autotuner found for this K (gemm_amd_emulation_counts). */
 int gemm_amd_dgemm_ozaki_crt(char transa, char transb, int m, int n, int k, double alpha,
                              const double *a, int lda, const double *b, int ldb, double beta,
                              double *c, int ldc, int nmoduli, void *stream);
 /* Bits kept per row/column: the largest b with k*2^(2b) <= M/4 (M = product of
  * the moduli), at most 56; -1 for invalid arguments. */
+/* Moduli (CRT form) and slices (slice form) used when the emulated DGEMMs
+ * are called with 0 for that count: per ceil(log2 K), the smallest counts
+ * whose max error the emulation autotuner (gemm_avx512f_amd.tuning,
+ * tune-emulation) measured at or below the native FP64 kernel's on inputs of
+ * dynamic range 2^0, 2^8 and 2^32 (no reference counterpart: the reference
+ * has no emulated product).  _set installs a table (33 entries each, index
+ * ceil(log2 K)); the library starts from the one compiled in. */
+int gemm_amd_emulation_counts(int k, int *nmoduli, int *nslices);
+int gemm_amd_emulation_counts_set(const int *moduli, const int *slices, int n);
+
 int gemm_amd_ozaki_crt_bits(int nmoduli, int k);
 /* The residue step alone (exposed for byte-exact checks): rows of the view of
  * x become integers rint(x * 2^(bits - e_r)) (e_r from frexp of the row's max),
@@ -245,11 +257,22 @@ int gemm_amd_bf16_split(const float *x, int64_t row_stride, int64_t k_stride, in
  * _replay relaunches the graph with no host-side work.  Problems are
  * scheduled largest first; results do not depend on the schedule.
  * _create returns NULL on an illegal argument (message in
- * gemm_amd_last_error()); _destroy synchronises the device first. */
+ * gemm_amd_last_error()); _destroy synchronises the device first.
+ * Each problem runs on one of two tile classes in the same launch: large
+ * tiles (FP32 and bf16 128x128, FP64 128x64) when M >= large_min and
+ * N >= large_min (FP64: large_min / 2), small 64x64 tiles otherwise.
+ * _create uses the default per dtype (bf16 96, FP64 128, FP32 no large
+ * tiles); _create_ex takes large_min (0 = that default, a value above every
+ * M keeps all problems on small tiles), which is what the grouped autotuner
+ * (gemm_avx512f_amd.tuning.tune_grouped) sweeps. */
 void *gemm_amd_grouped_create(int dtype, int count, const char *transa, const char *transb, const int *m,
                               const int *n, const int *k, const double *alpha, const void *const *a,
                               const int *lda, const void *const *b, const int *ldb, const double *beta,
                               void *const *c, const int *ldc);
+void *gemm_amd_grouped_create_ex(int dtype, int count, const char *transa, const char *transb, const int *m,
+                                 const int *n, const int *k, const double *alpha, const void *const *a,
+                                 const int *lda, const void *const *b, const int *ldb, const double *beta,
+                                 void *const *c, const int *ldc, int large_min);
 int gemm_amd_grouped_run(void *plan, void *stream);
 int gemm_amd_grouped_capture(void *plan, void *stream);
 int gemm_amd_grouped_replay(void *plan, void *stream);

@@ -125,3 +125,27 @@ def test_noalloc_dtrsm_rejects_before_device_work():
     assert L.gemm_amd_dtrsm_noalloc(b"R", b"L", b"N", b"N", 4, 32769, 1.0, None, 32769, None, 4, None) == -1
     assert "exceeds the maximum of 32768" in _lib.last_error()
     assert L.gemm_amd_dtrsm_noalloc(b"L", b"L", b"N", b"N", 0, 4, 1.0, None, 1, None, 1, None) == 0
+
+
+def test_emulation_counts_table():
+    # the tuned table compiled into the library: 15 moduli up to K = 512, 16
+    # beyond; 8 slices everywhere; installing a table is validated
+    from gemm_avx512f_amd import tuning
+    assert tuning.emulation_counts(64) == (15, 8)
+    assert tuning.emulation_counts(512) == (15, 8)
+    assert tuning.emulation_counts(513) == (16, 8)
+    assert tuning.emulation_counts(8192) == (16, 8)
+    bad = (ctypes.c_int * 33)(*([1] * 33))
+    ok = (ctypes.c_int * 33)(*([8] * 33))
+    assert _lib.lib().gemm_amd_emulation_counts_set(bad, ok, 33) < 0
+    assert _lib.lib().gemm_amd_emulation_counts(0, None, None) < 0
+
+
+def test_counts_tables_from_tuning_result():
+    from gemm_avx512f_amd import tuning
+    tuned = {64: {"moduli": 12, "slices": 6}, 1024: {"moduli": 14, "slices": 7}, 8192: {"moduli": 16, "slices": 8}}
+    mod, sl = tuning.counts_tables(tuned)
+    assert len(mod) == len(sl) == 33
+    assert mod[0] == mod[6] == 12 and mod[7] == mod[10] == 14 and mod[11] == mod[13] == 16
+    assert mod[14] == 16 and sl[32] == 8                      # beyond the tuned range: most accurate
+    assert sl[6] == 6 and sl[10] == 7

@@ -26,13 +26,22 @@ from tests.util import opmat
 
 pytestmark = pytest.mark.gpu
 
-# the grouped kernel's tile per dtype and the single-problem tile with the same shape and slab depth
-GROUP_TILE = {GROUPED_F32: 64, GROUPED_BF16: 64, GROUPED_F64: 64}
+# tile classes per dtype: small 64x64 tiles, large tiles for problems whose M
+# and N (FP64: N half of it) reach the plan's threshold, by default bf16 96,
+# FP64 128 and none for FP32
+SMALL_TILE = (64, 64)
+LARGE_TILE = {GROUPED_F32: (128, 128), GROUPED_BF16: (128, 128), GROUPED_F64: (128, 64)}
+DEFAULT_LARGE_MIN = {GROUPED_F32: 1 << 30, GROUPED_BF16: 96, GROUPED_F64: 128}
 
 
-def expected_tiles(dtype, problems):
-    t = GROUP_TILE[dtype]
-    return sum(-(-p["m"] // t) * -(-p["n"] // t) for p in problems)
+def expected_tiles(dtype, problems, large_min=None):
+    t = DEFAULT_LARGE_MIN[dtype] if not large_min else large_min
+    total = 0
+    for p in problems:
+        large = p["m"] >= t and p["n"] >= (t // 2 if dtype == GROUPED_F64 else t)
+        bm, bn = LARGE_TILE[dtype] if large else SMALL_TILE
+        total += -(-p["m"] // bm) * -(-p["n"] // bn)
+    return total
 
 
 SAME_SINGLE_TILE = {GROUPED_F32: 1, GROUPED_F64: 1}   # TILE_64x64
@@ -141,6 +150,28 @@ def test_grouped_bitwise_equals_single_gemm(dtype, shapes):
     for p in P:
         assert np.array_equal(p["c"].to_host_padded(), p["c_single"].to_host_padded()), \
             (p["transa"], p["transb"], p["m"], p["n"], p["k"])
+
+
+@pytest.mark.parametrize("dtype", [GROUPED_F32, GROUPED_BF16, GROUPED_F64], ids=["f32", "bf16", "f64"])
+def test_tile_classes_bitwise_equal(dtype):
+    # every problem on large tiles where it qualifies, against every problem on
+    # small tiles: same k order and epilogue, so the same bits
+    P = upload(random_problems(dtype, 40, seed=61 + dtype, lo=16, hi=700))
+    results = {}
+    for t in (32, 1 << 30):
+        for p in P:
+            p["c"].buf.upload(p["hc"])
+        g = GroupedGemm(dtype, P, large_min=t)
+        assert g.tiles == expected_tiles(dtype, P, large_min=t)
+        g.run()
+        synchronize()
+        results[t] = [p["c"].to_host_padded() for p in P]
+        g.destroy()
+    assert any(p["m"] >= 32 and p["n"] >= 32 for p in P)
+    for p, big, small in zip(P, results[32], results[1 << 30]):
+        assert np.array_equal(big, small), (p["transa"], p["transb"], p["m"], p["n"], p["k"])
+    for p, got in zip(P, results[32]):
+        oracle_check(dtype, p, got)
 
 
 @pytest.mark.parametrize("dtype", [GROUPED_F32, GROUPED_BF16], ids=["f32", "bf16"])

@@ -144,3 +144,21 @@ def test_long_k_reduction_and_checks():
     out = run_crt("N", "N", 40, 30, 20, 0.0, a[:40, :20].copy(order="F"), b[:20, :30].copy(order="F"), 2.0,
                   np.ones((40, 30), order="F"), 16)
     assert np.all(out == 2.0)
+
+
+def test_auto_counts_match_explicit():
+    """nmoduli = 0 / nslices = 0 take the tuned counts for K (16 moduli, 8
+    slices at K = 1500) and give the explicit calls' bits."""
+    from gemm_avx512f_amd import tuning
+    m, n, k = 96, 80, 1500
+    a, b, c, lda, ldb, ldc = gemm_case(np.float64, 71, "N", "N", m, n, k)
+    nm, ns = tuning.emulation_counts(k)
+    assert (nm, ns) == (16, 8)
+    assert np.array_equal(run_crt("N", "N", m, n, k, 1.0, a, b, 0.5, c, 0), run_crt("N", "N", m, n, k, 1.0, a, b, 0.5, c, nm))
+    dA, dB = DeviceMatrix.from_padded(a, a.shape[0]), DeviceMatrix.from_padded(b, b.shape[0])
+    outs = []
+    for count in (0, ns):
+        dC = DeviceMatrix.from_padded(c, m)
+        blas.dgemm_ozaki("N", "N", m, n, k, 1.0, dA, dB, 0.5, dC, nslices=count)
+        outs.append(dC.to_host_padded())
+    assert np.array_equal(outs[0], outs[1])
