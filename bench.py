@@ -33,6 +33,11 @@ PEAK_INT8_TOPS = 5000.0
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "dgemm_8192_nn_traffic.json")
 
 
+def _progress(msg):
+    """One line per leg on stderr (the JSON result is the only line on stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def _time(fn, stream, steps, warmup):
     from gemm_avx512f_amd.device import Event, synchronize
     for _ in range(warmup):
@@ -491,6 +496,7 @@ def main():
         synchronize()
         del src, dst
     variants = () if args.no_emulated else (("crt", args.nmoduli), ("crt", args.nmoduli - 1), ("slices", args.nslices))
+    _progress("native and emulated DGEMM %d^3" % size)
     legs = gpu_leg(size, args.steps, args.warmup, variants=variants)
     ms, wall = legs["native"]["ms"], legs["native"]["wall"]
     flops = 2.0 * size ** 3
@@ -533,6 +539,8 @@ def main():
     result["config"]["kernel_config"] = {"tile": ["128x128", "64x64", "128x64", "256x64"][tile.value],
                                          "splitk": splitk.value,
                                          "kernels_per_step": 1 if splitk.value == 1 else 2}
+    if not args.no_emulated:
+        _progress("accuracy against dynamic range")
     accuracy = accuracy_leg(size) if not args.no_emulated else None
     variants = []
     for e in legs["emulated"]:
@@ -563,16 +571,21 @@ def main():
     if accuracy is not None:
         result["accuracy_vs_range"] = accuracy
     if not args.no_dropin:
+        _progress("host-pointer dgemm_")
         a, b = legs["host_ab"]
         result["dropin_host_pointer"] = dropin_leg(size, a, b)
     del legs["host_ab"]
     if not args.no_sgemm:
+        _progress("SGEMM and bf16x")
         result["sgemm"] = sgemm_leg(size, args.steps, args.warmup)
     if not args.no_grouped:
+        _progress("grouped and zgemm3m batch")
         result["batching"] = grouped_leg()
     if not args.no_configs:
+        _progress("SGEMM 1024, DGEMM 4096, DTRSM, LU")
         result["configs"] = configs_leg()
     if not args.no_cpu:
+        _progress("reference CPU dgemm_")
         try:
             result["cpu_baseline"] = cpu_leg(args.cpu_size or size)
         except Exception as exc:  # reported, never silently replaced

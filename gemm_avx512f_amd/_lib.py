@@ -99,6 +99,7 @@ SIGNATURES = {
     "gemm_amd_picker_load": (c_int, [c_vp, c_int]),
     "gemm_amd_picker_set_mode": (c_int, [c_int]),
     "gemm_amd_picker_choose": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "gemm_amd_picker_features": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp]),
     "gemm_amd_tune_table_set": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "gemm_amd_tune_table_clear": (c_int, []),
     "gemm_amd_mlp_forward": (c_int, [c_vp, c_vp, c_int, c_vp, c_vp]),

@@ -37,7 +37,8 @@ int gemm_device(const char *name, char ta, char tb, int m, int n, int k, T alpha
     if (m == 0 || n == 0 || ((alpha == T(0) || k == 0) && beta == T(1))) return 0;
     if (int st = init_status()) return st;
     GemmConfig cfg = force ? *force : pick_gemm_config(sizeof(T) == 8 ? DT_F64 : DT_F32, !notrans(ta),
-                                                       !notrans(tb), m, n, k, lda - (notrans(ta) ? m : k), rt().cus);
+                                                       !notrans(tb), m, n, k, lda - (notrans(ta) ? m : k),
+                                                       ldb - (notrans(tb) ? k : n), rt().cus);
     const size_t wbytes = gemm_workspace_bytes(m, n, k, cfg, (int)sizeof(T));
     if (!wbytes)
         return hip_status((hipError_t)launch<T>(!notrans(ta), !notrans(tb), m, n, k, alpha, a, lda, b, ldb, beta, c,
@@ -107,7 +108,7 @@ void gemm_host(const char *name, const char *transa, const char *transb, const i
     if (product) {
         // the staged copies are packed (leading dimension = rows)
         GemmConfig cfg = pick_gemm_config(sizeof(T) == 8 ? DT_F64 : DT_F32, !notrans(ta), !notrans(tb), m, n, k, 0,
-                                          R.cus);
+                                          0, R.cus);
         const size_t wbytes = gemm_workspace_bytes(m, n, k, cfg, (int)sizeof(T));
         if (wbytes && (rc = buf(BUF_WORK).ensure(wbytes))) {
             fprintf(stderr, "%s: %s\n", name, get_error());

@@ -21,7 +21,7 @@ int run3m(char ta, char tb, int m, int n, int k, const T *alpha, const T *a, int
     const int nrowa = notrans(ta) ? m : k;
     const GemmConfig cfg = force ? *force
                                  : pick_gemm_config(sizeof(T) == 8 ? DT_C64 : DT_C32, !notrans(ta), !notrans(tb), m,
-                                                    n, k, lda - nrowa, rt().cus);
+                                                    n, k, lda - nrowa, ldb - (notrans(tb) ? k : n), rt().cus);
     void *gws = lease.get(WS_GEMM, gemm_workspace_bytes(m, n, k, cfg, (int)sizeof(T)), &st);
     if (st) return st;
     int e;

@@ -132,11 +132,11 @@ int gemm_amd_pick_config(int op, char transa, char transb, int m, int n, int k, 
     } else if (mode == 1) {
         int choice = 0;
         const char t1 = ta ? 'T' : 'N', t2 = tb ? 'T' : 'N';
-        int r = picker_choose(1, &op, &t1, &t2, &m, &n, &k, &lda_pad, &choice);
+        int r = picker_choose(1, &op, &t1, &t2, &m, &n, &k, &lda_pad, rt().cus, &choice);
         if (r) return hip_status((hipError_t)r, "picker (no weights loaded?)");
         c = tune_candidate(choice);
     } else {
-        c = pick_gemm_config(dt, ta, tb, m, n, k, lda_pad, rt().cus);
+        c = pick_gemm_config(dt, ta, tb, m, n, k, lda_pad, 0, rt().cus);   // ldb as in the sweep: rows of op(B)
     }
     *tile = c.tile;
     *splitk = c.splitk;
@@ -162,6 +162,24 @@ int gemm_amd_picker_set_mode(int mode) {
     return 0;
 }
 
+int gemm_amd_picker_features(int count, const int *op, const char *transa, const char *transb, const int *m,
+                             const int *n, const int *k, const int *lda_pad, int cus, float *out) {
+    if (count < 0 || cus < 1 || (count > 0 && !out)) {
+        set_error("gemm_amd_picker_features: count >= 0, cus >= 1 and an output array");
+        return -(int)hipErrorInvalidValue;
+    }
+    for (int i = 0; i < count; ++i) {
+        if (!op_ok(op[i]) || m[i] < 1 || n[i] < 1 || k[i] < 1 || lda_pad[i] < 0) {
+            set_error("gemm_amd_picker_features: op 0..2, m, n, k >= 1 and lda_pad >= 0 for every shape");
+            return -(int)hipErrorInvalidValue;
+        }
+        const bool ta = !notrans(transa[i]), tb = !notrans(transb[i]);
+        mlp_features(op[i], ta, tb, m[i], n[i], k[i], lda_pad[i],
+                     gemm_vec_path(op[i], ta, tb, m[i], n[i], k[i], lda_pad[i], 0), cus, out + (size_t)i * MLP_IN);
+    }
+    return 0;
+}
+
 int gemm_amd_picker_choose(int count, const int *op, const char *transa, const char *transb, const int *m,
                            const int *n, const int *k, const int *lda_pad, int *choice) {
     if (count < 0) return -(int)hipErrorInvalidValue;
@@ -171,7 +189,7 @@ int gemm_amd_picker_choose(int count, const int *op, const char *transa, const c
             return -(int)hipErrorInvalidValue;
         }
     if (int st = init_status()) return st;
-    return hip_status((hipError_t)picker_choose(count, op, transa, transb, m, n, k, lda_pad, choice),
+    return hip_status((hipError_t)picker_choose(count, op, transa, transb, m, n, k, lda_pad, rt().cus, choice),
                       "picker (no weights loaded?)");
 }
 

@@ -23,6 +23,11 @@ int tile_bm(int tile, int elem_bytes);
 int tile_bn(int tile, int elem_bytes);
 int tile_bk(int tile, int elem_bytes);
 
+// Number of K partitions a split-K request of `splitk` runs as: each
+// partition is a whole number of 32-deep slabs, so for shallow K there may be
+// fewer than asked.
+int gemm_k_partitions(int k, int splitk);
+
 // Workspace bytes a config needs (split-K partials), 0 if none.
 size_t gemm_workspace_bytes(int m, int n, int k, const GemmConfig &cfg, int elem_bytes);
 

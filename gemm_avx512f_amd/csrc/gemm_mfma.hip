@@ -230,6 +230,21 @@ int tile_bm(int t, int elem_bytes) { return tile_shape(t, elem_bytes).BM; }
 int tile_bn(int t, int elem_bytes) { return tile_shape(t, elem_bytes).BN; }
 int tile_bk(int t, int elem_bytes) { return tile_shape(t, elem_bytes).BK; }
 
+// K depth of each partition: ceil(K / splitk) rounded up to the 32-deep slab
+// every tile shape divides (partitions start on a slab boundary).
+static int k_chunk(int k, int splitk) {
+    constexpr int BK = 32;
+    const int s = splitk > 1 ? splitk : 1;
+    const int chunk = (k + s - 1) / s;
+    return (chunk + BK - 1) / BK * BK;
+}
+
+int gemm_k_partitions(int k, int splitk) {
+    if (k <= 0) return 1;
+    const int chunk = k_chunk(k, splitk);
+    return (k + chunk - 1) / chunk;
+}
+
 size_t gemm_workspace_bytes(int m, int n, int, const GemmConfig &cfg, int elem_bytes) {
     if (cfg.splitk <= 1) return 0;
     return (size_t)cfg.splitk * (size_t)m * (size_t)n * (size_t)elem_bytes;
@@ -272,7 +287,6 @@ static hipError_t launch_gemm_t(bool ta, bool tb, int m, int n, int k, T alpha, 
         return hipGetLastError();
     }
     const int tile = (cfg.tile >= 0 && cfg.tile < TILE_COUNT) ? cfg.tile : TILE_128x128;
-    constexpr int BK = 32;   // split-K partitions start on a multiple of every tile's slab depth
     constexpr int VW = 16 / (int)sizeof(T);
     GemmParams<T> p;
     p.M = m; p.N = n; p.K = k;
@@ -281,11 +295,8 @@ static hipError_t launch_gemm_t(bool ta, bool tb, int m, int n, int k, T alpha, 
     const TileShape &ts = tile_shape(tile, (int)sizeof(T));
     p.tiles_m = (m + ts.BM - 1) / ts.BM;
     p.tiles_n = (n + ts.BN - 1) / ts.BN;
-    int splits = cfg.splitk > 1 ? cfg.splitk : 1;
-    int kchunk = (k + splits - 1) / splits;
-    kchunk = (kchunk + BK - 1) / BK * BK;
-    splits = (k + kchunk - 1) / kchunk;
-    p.kchunk = kchunk;
+    const int splits = gemm_k_partitions(k, cfg.splitk);
+    p.kchunk = k_chunk(k, cfg.splitk);
     p.work = (splits > 1) ? static_cast<T *>(work) : nullptr;
     if (splits > 1 && !work) return hipErrorInvalidValue;
 

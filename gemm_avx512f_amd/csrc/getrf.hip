@@ -310,7 +310,7 @@ struct Lu {
 
     void gemm(int mm, int nn, int kk, const double *a, const double *b, double *c) {
         if (err || mm <= 0 || nn <= 0 || kk <= 0) return;
-        GemmConfig cfg = pick_gemm_config(DT_F64, false, false, mm, nn, kk, (int)(lda - mm), cus);
+        GemmConfig cfg = pick_gemm_config(DT_F64, false, false, mm, nn, kk, (int)(lda - mm), (int)(lda - kk), cus);
         if (gemm_workspace_bytes(mm, nn, kk, cfg, 8) > gemm_ws_bytes) cfg.splitk = 1;
         err = launch_dgemm(false, false, mm, nn, kk, -1.0, a, lda, b, lda, 1.0, c, lda, cfg, gemm_ws, s);
     }

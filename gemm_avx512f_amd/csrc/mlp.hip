@@ -7,11 +7,12 @@
 // order: results are bitwise reproducible run to run, which the training
 // tests rely on.  There are no atomics.
 #include "mlp.hpp"
+#include "gemm_kernels.hpp"
 #include <cmath>
 
 namespace gemm_amd {
 
-void mlp_features(int op, bool ta, bool tb, int m, int n, int k, int lda_pad, float *f) {
+void mlp_features(int op, bool ta, bool tb, int m, int n, int k, int lda_pad, bool vec, int cus, float *f) {
     f[0] = (float)((std::log2((double)(m > 0 ? m : 1)) - 9.0) / 5.0);
     f[1] = (float)((std::log2((double)(n > 0 ? n : 1)) - 9.0) / 5.0);
     f[2] = (float)((std::log2((double)(k > 0 ? k : 1)) - 9.0) / 5.0);
@@ -21,6 +22,19 @@ void mlp_features(int op, bool ta, bool tb, int m, int n, int k, int lda_pad, fl
     f[6] = op == 1 ? 1.0f : 0.0f;
     f[7] = op == 2 ? 1.0f : 0.0f;
     f[8] = (float)(((lda_pad % 64) + 64) % 64) / 64.0f;
+    f[9] = vec ? 1.0f : 0.0f;
+    const int eb = op == 1 ? 8 : 4;   // the CGEMM3M products are FP32 GEMMs
+    const int64_t mm = m > 0 ? m : 1, nn = n > 0 ? n : 1, cu = cus > 0 ? cus : 1;
+    for (int c = 0; c < 20; ++c) {
+        const int tile = c / 5, bm = tile_bm(tile, eb), bn = tile_bn(tile, eb);
+        const int64_t w = ((mm + bm - 1) / bm) * ((nn + bn - 1) / bn) * gemm_k_partitions(k, 1 << (c % 5));
+        const int64_t waves = (w + cu - 1) / cu;
+        f[10 + c] = (float)((double)w / (double)(waves * cu));
+    }
+    for (int t = 0; t < 4; ++t) {
+        const int64_t bm = tile_bm(t, eb), bn = tile_bn(t, eb);
+        f[30 + t] = (float)((double)(mm * nn) / (double)(((mm + bm - 1) / bm) * bm * ((nn + bn - 1) / bn) * bn));
+    }
 }
 
 // Z[b][o] = bias[o] + sum_i X[b][i] W[i][o]  (ReLU optional)

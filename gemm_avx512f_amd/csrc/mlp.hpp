@@ -1,12 +1,22 @@
-// The configuration picker's network: a 3-layer perceptron (9 -> 64 -> 64 ->
+// The configuration picker's network: a 3-layer perceptron (34 -> 64 -> 64 ->
 // 20, ReLU after the two hidden layers, FP32) trained and run by the HIP
 // kernels of mlp.hip.
 //
 // Parameters are one flat FP32 array, row-major weights [in][out]:
-//   W1[9][64] b1[64] W2[64][64] b2[64] W3[64][20] b3[20]   (6100 floats)
+//   W1[34][64] b1[64] W2[64][64] b2[64] W3[64][20] b3[20]   (7700 floats)
 // Input features of one GEMM problem (mlp_features):
-//   (log2 M - 9)/5, (log2 N - 9)/5, (log2 K - 9)/5, transA, transB,
-//   op one-hot (SGEMM, DGEMM, CGEMM3M), (lda - rows of op(A) storage) mod 64 / 64
+//   [0..8]   (log2 M - 9)/5, (log2 N - 9)/5, (log2 K - 9)/5, transA, transB,
+//            op one-hot (SGEMM, DGEMM, CGEMM3M), (lda - rows of op(A) storage) mod 64 / 64
+//   [9]      1 when the kernels take their 16-byte vector-load path (leading
+//            dimensions a multiple of 16 bytes), else 0
+//   [10..29] per candidate c (tile c / 5, split-K 2^(c % 5)): how full the
+//            last wave of workgroups is, w / (ceil(w / CUs) CUs) with
+//            w = tiles x K partitions
+//   [30..33] per tile shape: the share of the padded tile grid that lies
+//            inside C, M N / (ceil(M/BM) BM ceil(N/BN) BN)
+// The last two groups carry what log2 sizes cannot: a 2048^2 FP32 product
+// fills the 256 CUs with 128x128 tiles exactly, a 2185^2 one needs a second,
+// mostly empty wave.
 // Output c is the predicted log2(time of candidate c / time of the best
 // candidate) for the 20 tuning candidates; the picker takes the argmin over
 // the candidates valid for the shape.
@@ -16,7 +26,7 @@
 
 namespace gemm_amd {
 
-constexpr int MLP_IN = 9, MLP_HID = 64, MLP_OUT = 20;
+constexpr int MLP_IN = 34, MLP_HID = 64, MLP_OUT = 20;
 constexpr int MLP_OFF_W1 = 0;
 constexpr int MLP_OFF_B1 = MLP_OFF_W1 + MLP_IN * MLP_HID;
 constexpr int MLP_OFF_W2 = MLP_OFF_B1 + MLP_HID;
@@ -25,7 +35,8 @@ constexpr int MLP_OFF_W3 = MLP_OFF_B2 + MLP_HID;
 constexpr int MLP_OFF_B3 = MLP_OFF_W3 + MLP_HID * MLP_OUT;
 constexpr int MLP_NPARAMS = MLP_OFF_B3 + MLP_OUT;
 
-void mlp_features(int op, bool ta, bool tb, int m, int n, int k, int lda_pad, float *f);
+// `vec`: the 16-byte load path applies (gemm_vec_path); `cus`: compute units.
+void mlp_features(int op, bool ta, bool tb, int m, int n, int k, int lda_pad, bool vec, int cus, float *f);
 
 // Device-side building blocks (all pointers device memory, row-major
 // [batch][features]).  Scratch `ws` holds the two hidden activations

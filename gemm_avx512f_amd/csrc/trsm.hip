@@ -293,8 +293,9 @@ struct TrsmCtx {
 
     // The trailing updates take the configuration every DGEMM call gets (the
     // learned picker, or its fallback), limited to the split-K workspace.
-    GemmConfig cfg(bool ta, int mm, int nn, int kk, int64_t la) const {
-        GemmConfig c = pick_gemm_config(DT_F64, ta, false, mm, nn, kk, (int)(la - (ta ? kk : mm)), cus);
+    GemmConfig cfg(bool ta, bool tb, int mm, int nn, int kk, int64_t la, int64_t lb) const {
+        GemmConfig c = pick_gemm_config(DT_F64, ta, tb, mm, nn, kk, (int)(la - (ta ? kk : mm)),
+                                        (int)(lb - (tb ? nn : kk)), cus);
         if (gemm_workspace_bytes(mm, nn, kk, c, 8) > gemm_work_bytes) c.splitk = 1;
         return c;
     }
@@ -302,7 +303,7 @@ struct TrsmCtx {
     void gemm(bool ta, bool tb, int mm, int nn, int kk, double alpha, const double *a, int64_t la, const double *b,
               int64_t lb, double beta, double *c, int64_t lc) {
         if (err || mm <= 0 || nn <= 0) return;
-        err = launch_dgemm(ta, tb, mm, nn, kk, alpha, a, la, b, lb, beta, c, lc, cfg(ta, mm, nn, kk, la), gemm_work,
+        err = launch_dgemm(ta, tb, mm, nn, kk, alpha, a, la, b, lb, beta, c, lc, cfg(ta, tb, mm, nn, kk, la, lb), gemm_work,
                            s);
     }
 

@@ -105,21 +105,22 @@ GemmConfig heuristic_gemm_config(int dtype, int m, int n, int k, int cus) {
 namespace {
 
 struct Key {
-    int op, ta, tb, m, n, k, pad;
+    int op, ta, tb, m, n, k, pad, vec;
     bool operator==(const Key &o) const {
-        return op == o.op && ta == o.ta && tb == o.tb && m == o.m && n == o.n && k == o.k && pad == o.pad;
+        return op == o.op && ta == o.ta && tb == o.tb && m == o.m && n == o.n && k == o.k && pad == o.pad &&
+               vec == o.vec;
     }
 };
 struct KeyHash {
     size_t operator()(const Key &x) const {
         uint64_t h = 1469598103934665603ull;
-        for (int v : {x.op, x.ta, x.tb, x.m, x.n, x.k, x.pad}) h = (h ^ (uint32_t)v) * 1099511628211ull;
+        for (int v : {x.op, x.ta, x.tb, x.m, x.n, x.k, x.pad, x.vec}) h = (h ^ (uint32_t)v) * 1099511628211ull;
         return (size_t)h;
     }
 };
 
-Key make_key(int op, bool ta, bool tb, int m, int n, int k, int pad) {
-    return Key{op, ta ? 1 : 0, tb ? 1 : 0, m, n, k, ((pad % 64) + 64) % 64};
+Key make_key(int op, bool ta, bool tb, int m, int n, int k, int pad, bool vec) {
+    return Key{op, ta ? 1 : 0, tb ? 1 : 0, m, n, k, ((pad % 64) + 64) % 64, vec ? 1 : 0};
 }
 
 struct Picker {
@@ -242,16 +243,18 @@ int picker_mode() {
 }
 
 int picker_choose(int count, const int *op, const char *ta, const char *tb, const int *m, const int *n, const int *k,
-                  const int *lda_pad, int *choice) {
+                  const int *lda_pad, int cus, int *choice) {
     Picker &P = picker();
     std::lock_guard<std::mutex> lk(P.mu);
     decide_defaults_locked(P);
     if (!P.loaded) return (int)hipErrorNotInitialized;
     if (count <= 0) return 0;
     std::vector<float> feat((size_t)count * MLP_IN), y((size_t)count * MLP_OUT);
-    for (int i = 0; i < count; ++i)
-        mlp_features(op[i], !(ta[i] == 'N' || ta[i] == 'n'), !(tb[i] == 'N' || tb[i] == 'n'), m[i], n[i], k[i],
-                     lda_pad[i], &feat[(size_t)i * MLP_IN]);
+    for (int i = 0; i < count; ++i) {
+        const bool a = !(ta[i] == 'N' || ta[i] == 'n'), b = !(tb[i] == 'N' || tb[i] == 'n');
+        mlp_features(op[i], a, b, m[i], n[i], k[i], lda_pad[i], gemm_vec_path(op[i], a, b, m[i], n[i], k[i],
+                     lda_pad[i], 0), cus, &feat[(size_t)i * MLP_IN]);
+    }
     if (int r = infer_locked(P, count, feat.data(), y.data())) return r;
     for (int i = 0; i < count; ++i) choice[i] = argmin_valid(op[i], m[i], n[i], k[i], &y[(size_t)i * MLP_OUT]);
     return 0;
@@ -260,13 +263,14 @@ int picker_choose(int count, const int *op, const char *ta, const char *tb, cons
 void table_set(int op, bool ta, bool tb, int m, int n, int k, int lda_pad, int choice) {
     Picker &P = picker();
     std::lock_guard<std::mutex> lk(P.mu);
-    P.table[make_key(op, ta, tb, m, n, k, lda_pad)] = tune_candidate(choice);
+    P.table[make_key(op, ta, tb, m, n, k, lda_pad, gemm_vec_path(op, ta, tb, m, n, k, lda_pad, 0))] =
+        tune_candidate(choice);
 }
 
 bool table_get(int op, bool ta, bool tb, int m, int n, int k, int lda_pad, GemmConfig *out) {
     Picker &P = picker();
     std::lock_guard<std::mutex> lk(P.mu);
-    auto it = P.table.find(make_key(op, ta, tb, m, n, k, lda_pad));
+    auto it = P.table.find(make_key(op, ta, tb, m, n, k, lda_pad, gemm_vec_path(op, ta, tb, m, n, k, lda_pad, 0)));
     if (it == P.table.end()) return false;
     *out = it->second;
     return true;
@@ -283,7 +287,14 @@ static thread_local bool t_pick_without_device = false;
 PickWithoutDevice::PickWithoutDevice() : prev_(t_pick_without_device) { t_pick_without_device = true; }
 PickWithoutDevice::~PickWithoutDevice() { t_pick_without_device = prev_; }
 
-GemmConfig pick_gemm_config(int dtype, bool ta, bool tb, int m, int n, int k, int lda_pad, int cus) {
+bool gemm_vec_path(int op, bool ta, bool tb, int m, int n, int k, int lda_pad, int ldb_pad) {
+    if (op == OP_CGEMM3M) return m % 4 == 0 && k % 4 == 0;
+    const int vw = op == OP_DGEMM ? 2 : 4;   // elements per 16 bytes
+    const long lda = (long)(ta ? k : m) + lda_pad, ldb = (long)(tb ? n : k) + ldb_pad;
+    return lda % vw == 0 && ldb % vw == 0;
+}
+
+GemmConfig pick_gemm_config(int dtype, bool ta, bool tb, int m, int n, int k, int lda_pad, int ldb_pad, int cus) {
     static const char *ft = getenv("GEMM_AMD_FORCE_TILE");
     static const char *fs = getenv("GEMM_AMD_FORCE_SPLITK");
     GemmConfig c = heuristic_gemm_config(dtype, m, n, k, cus);
@@ -302,7 +313,8 @@ GemmConfig pick_gemm_config(int dtype, bool ta, bool tb, int m, int n, int k, in
     if (op < 0) return c;
     Picker &P = picker();
     std::lock_guard<std::mutex> lk(P.mu);
-    const Key key = make_key(op, ta, tb, m, n, k, lda_pad);
+    const bool vec = gemm_vec_path(op, ta, tb, m, n, k, lda_pad, ldb_pad);
+    const Key key = make_key(op, ta, tb, m, n, k, lda_pad, vec);
     auto it = P.table.find(key);
     if (it != P.table.end()) return it->second;
     if (t_pick_without_device) {
@@ -316,7 +328,7 @@ GemmConfig pick_gemm_config(int dtype, bool ta, bool tb, int m, int n, int k, in
     auto ct = P.cache.find(key);
     if (ct != P.cache.end()) return ct->second;
     float feat[MLP_IN], y[MLP_OUT];
-    mlp_features(op, ta, tb, m, n, k, lda_pad, feat);
+    mlp_features(op, ta, tb, m, n, k, lda_pad, vec, cus, feat);
     if (infer_locked(P, 1, feat, y)) return c;   // inference failure: keep the heuristic
     const GemmConfig g = tune_candidate(argmin_valid(op, m, n, k, y));
     P.cache[key] = g;

@@ -31,7 +31,14 @@ bool tune_valid(int op, int m, int n, int k, const GemmConfig &c);
 
 enum PickMode : int { PICK_HEURISTIC = 0, PICK_MLP = 1 };
 
-GemmConfig pick_gemm_config(int dtype, bool ta, bool tb, int m, int n, int k, int lda_pad, int cus);
+// lda_pad, ldb_pad: leading dimension minus the rows of op(A), op(B) storage.
+GemmConfig pick_gemm_config(int dtype, bool ta, bool tb, int m, int n, int k, int lda_pad, int ldb_pad, int cus);
+
+// Whether the GEMM kernels take their 16-byte vector-load path for this
+// problem (both leading dimensions a multiple of 16 bytes; operands from
+// hipMalloc are aligned).  The CGEMM3M products read packed planes
+// (leading dimensions M and K).
+bool gemm_vec_path(int op, bool ta, bool tb, int m, int n, int k, int lda_pad, int ldb_pad);
 
 // While one lives on a thread, pick_gemm_config on that thread does no device
 // work: a shape that neither the sweep table nor the picker's cache holds gets
@@ -56,9 +63,9 @@ void picker_set_mode(int mode);
 int picker_mode();
 // Batched HIP inference: choice[i] = index of the predicted-fastest valid candidate.
 int picker_choose(int count, const int *op, const char *ta, const char *tb, const int *m, const int *n, const int *k,
-                  const int *lda_pad, int *choice);
+                  const int *lda_pad, int cus, int *choice);
 
-// Exhaustive-sweep table.
+// Exhaustive-sweep table (shapes as swept: ldb = rows of op(B)).
 void table_set(int op, bool ta, bool tb, int m, int n, int k, int lda_pad, int choice);
 bool table_get(int op, bool ta, bool tb, int m, int n, int k, int lda_pad, GemmConfig *out);
 void table_clear();

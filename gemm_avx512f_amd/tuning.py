@@ -4,7 +4,7 @@
   of one problem on the GPU; the sweep is exhaustive, so it is both the
   fallback for shapes it has seen (:func:`table_set`) and the judge of the
   picker's regret.
-* The picker is a 9-64-64-20 ReLU perceptron predicting, per candidate,
+* The picker is a 34-64-64-20 ReLU perceptron predicting, per candidate,
   log2(time / best time).  It is trained on the GPU by the library's own HIP
   kernels (:func:`train`), and run by its HIP inference kernel
   (:func:`picker_choose`, and inside every GEMM call for unseen shapes).
@@ -31,7 +31,7 @@ from . import _lib
 OP_SGEMM, OP_DGEMM, OP_CGEMM3M = 0, 1, 2
 OP_NAMES = {OP_SGEMM: "sgemm", OP_DGEMM: "dgemm", OP_CGEMM3M: "cgemm3m"}
 NCFG = 20
-MLP_IN, MLP_HID, MLP_OUT = 9, 64, 20
+MLP_IN, MLP_HID, MLP_OUT = 34, 64, 20
 MLP_NPARAMS = MLP_IN * MLP_HID + MLP_HID + MLP_HID * MLP_HID + MLP_HID + MLP_HID * MLP_OUT + MLP_OUT
 TARGET_CLIP = 3.0          # targets are log2(t / t_best) clipped at 8x slower
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -108,7 +108,53 @@ def sample_shapes(seed, count):
     return s
 
 
-def features(shapes):
+def sample_shapes_aligned(seed, count):
+    """Seeded GEMM problems of the sizes programs and benchmarks favour: each
+    of M, N, K a power of two in [64, 16384] (half the draws) or a multiple
+    of 64, log-uniform over the same range; op balanced; random transposes;
+    lda padding 0 in 3 of 4 problems (the 16-byte load path), else 1..63.
+    Here the tile grid often fills whole waves of workgroups exactly, which
+    is what the wave-fill features resolve."""
+    rng = np.random.default_rng(seed)
+    s = np.zeros(count, dtype=SHAPE_DTYPE)
+    s["op"] = np.arange(count) % 3
+    for key in ("m", "n", "k"):
+        pow2 = 2 ** rng.integers(6, 15, count)
+        mult = np.maximum(64, np.rint(np.exp(rng.uniform(np.log(64), np.log(16384), count)) / 64) * 64)
+        s[key] = np.where(rng.uniform(size=count) < 0.5, pow2, mult).astype(np.int32)
+    s["ta"], s["tb"] = rng.integers(0, 2, count), rng.integers(0, 2, count)
+    pad = rng.integers(1, 64, count)
+    s["pad"] = np.where(rng.uniform(size=count) < 0.75, 0, pad)
+    return s
+
+
+CUS = 256          # MI355X compute units (the library passes the device's own count)
+# macro-tile (BM, BN) of the four tile shapes, FP32 and FP64 (gemm_mfma.hip kTilesF32 / kTilesF64)
+TILE_BMBN = {4: ((128, 128), (64, 64), (128, 64), (256, 128)), 8: ((128, 128), (64, 64), (128, 64), (256, 64))}
+
+
+def k_partitions(k, splitk):
+    """K partitions a split-K request runs as (each a whole number of 32-deep slabs)."""
+    k = np.asarray(k, dtype=np.int64)
+    chunk = (k + splitk - 1) // splitk
+    chunk = (chunk + 31) // 32 * 32
+    return np.where(k > 0, (k + chunk - 1) // np.maximum(chunk, 1), 1)
+
+
+def vec_path(shapes, ldb_pad=0):
+    """1 where the kernels take the 16-byte vector-load path (library's gemm_vec_path;
+    sweeps store B with ldb = rows of op(B))."""
+    s = shapes
+    op, m, n, k = (s[f].astype(np.int64) for f in ("op", "m", "n", "k"))
+    lda = np.where(s["ta"] != 0, k, m) + s["pad"]
+    ldb = np.where(s["tb"] != 0, n, k) + ldb_pad
+    vw = np.where(op == OP_DGEMM, 2, 4)
+    real = (lda % vw == 0) & (ldb % vw == 0)
+    planes = (m % 4 == 0) & (k % 4 == 0)        # CGEMM3M products read packed planes (ld M, K)
+    return np.where(op == OP_CGEMM3M, planes, real)
+
+
+def features(shapes, cus=CUS):
     """The network's inputs (same formula as the library's mlp_features)."""
     s = shapes
     f = np.zeros((len(s), MLP_IN), dtype=np.float64)
@@ -120,6 +166,18 @@ def features(shapes):
     for o in range(3):
         f[:, 5 + o] = s["op"] == o
     f[:, 8] = (s["pad"] % 64) / 64.0
+    f[:, 9] = vec_path(s)
+    m = np.maximum(s["m"].astype(np.int64), 1)
+    n = np.maximum(s["n"].astype(np.int64), 1)
+    f64 = s["op"] == OP_DGEMM
+    for t in range(4):
+        bm = np.where(f64, TILE_BMBN[8][t][0], TILE_BMBN[4][t][0])
+        bn = np.where(f64, TILE_BMBN[8][t][1], TILE_BMBN[4][t][1])
+        tm, tn = (m + bm - 1) // bm, (n + bn - 1) // bn
+        for j in range(5):
+            w = tm * tn * k_partitions(s["k"], 1 << j)
+            f[:, 10 + 5 * t + j] = w / (((w + cus - 1) // cus) * cus)
+        f[:, 30 + t] = (m * n) / (tm * bm * tn * bn)
     return f.astype(np.float32)
 
 
@@ -170,6 +228,15 @@ def _shape_args(shapes):
     ta = ctypes.create_string_buffer(b"".join(b"T" if v else b"N" for v in s["ta"]) or b"N")
     tb = ctypes.create_string_buffer(b"".join(b"T" if v else b"N" for v in s["tb"]) or b"N")
     return [ints(s["op"]), ta, tb, ints(s["m"]), ints(s["n"]), ints(s["k"]), ints(s["pad"])]
+
+
+def library_features(shapes, cus=CUS):
+    """The library's own feature rows for these shapes (host computation, no GPU)."""
+    out = np.zeros((len(shapes), MLP_IN), dtype=np.float32)
+    args = _shape_args(shapes)
+    _lib.check(_lib.lib().gemm_amd_picker_features(len(shapes), *args, cus, out.ctypes.data_as(ctypes.c_void_p)),
+               "picker_features")
+    return out
 
 
 def picker_load(params):
@@ -460,6 +527,7 @@ def _main(argv):
     s.add_argument("--count", type=int, required=True)
     s.add_argument("--out", required=True)
     s.add_argument("--ops", type=int, nargs="+", default=None, help="sweep only these ops (others stay -2)")
+    s.add_argument("--aligned", action="store_true", help="shapes from sample_shapes_aligned")
     t = sub.add_parser("train")
     t.add_argument("--data", nargs="+", required=True)
     t.add_argument("--out", required=True)
@@ -524,10 +592,11 @@ def _main(argv):
         return 0
     if a.cmd == "sweep":
         total = a.total or (a.start + a.count)
-        shapes = sample_shapes(a.seed, total)[a.start:a.start + a.count]
+        sampler = sample_shapes_aligned if a.aligned else sample_shapes
+        shapes = sampler(a.seed, total)[a.start:a.start + a.count]
         log("sweeping %d shapes (seed %d, %d..%d)" % (len(shapes), a.seed, a.start, a.start + len(shapes)))
         ms = sweep_shapes(shapes, log=log, ops=set(a.ops) if a.ops else None)
-        np.savez_compressed(a.out, shapes=shapes, ms=ms, seed=a.seed, start=a.start)
+        np.savez_compressed(a.out, shapes=shapes, ms=ms, seed=a.seed, start=a.start, aligned=int(a.aligned))
         log("wrote " + a.out)
     else:
         shapes, ms = load_dataset(a.data)

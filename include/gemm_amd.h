@@ -302,18 +302,23 @@ int gemm_amd_tune_sweep(int op, char transa, char transb, int m, int n, int k, i
  * (environment override, sweep table, picker when enabled, heuristic). */
 int gemm_amd_pick_config(int op, char transa, char transb, int m, int n, int k, int lda_pad, int mode, int *tile,
                          int *splitk);
-/* The learned picker: a 9-64-64-20 ReLU perceptron (6100 FP32 parameters,
+/* The learned picker: a 34-64-64-20 ReLU perceptron (7700 FP32 parameters,
  * flat layout W1 b1 W2 b2 W3 b3, weights [in][out] row-major) run by a HIP
  * inference kernel.  Trained weights are compiled in; _load replaces them. */
 int gemm_amd_picker_load(const float *params, int nparams);
 int gemm_amd_picker_set_mode(int mode);   /* 0 heuristic, 1 learned picker */
 int gemm_amd_picker_choose(int count, const int *op, const char *transa, const char *transb, const int *m,
                            const int *n, const int *k, const int *lda_pad, int *choice);
+/* The network's 34 input features of each shape (row-major out[count][34]) for
+ * a device of `cus` compute units, with ldb = rows of op(B) as in the sweep.
+ * Host-only computation: needs no GPU. */
+int gemm_amd_picker_features(int count, const int *op, const char *transa, const char *transb, const int *m,
+                             const int *n, const int *k, const int *lda_pad, int cus, float *out);
 /* Exact-shape results of the exhaustive sweep; they take precedence over the picker. */
 int gemm_amd_tune_table_set(int count, const int *op, const char *transa, const char *transb, const int *m,
                             const int *n, const int *k, const int *lda_pad, const int *choice);
 int gemm_amd_tune_table_clear(void);
-/* Training kernels (device pointers, row-major [batch][9] inputs, [batch][20]
+/* Training kernels (device pointers, row-major [batch][34] inputs, [batch][20]
  * targets and masks).  loss = sum(mask*(y-t)^2)/(batch*20). */
 int gemm_amd_mlp_forward(const float *params, const float *x, int batch, float *y, void *stream);
 int gemm_amd_mlp_grad(const float *params, const float *x, const float *t, const float *mask, int batch, float *grad,

@@ -90,7 +90,7 @@ void oracle_dgemm_blocked(char transa, char transb, int m, int n, int k, double 
 int oracle_dgetrf(int m, int n, double *a, int lda, int *ipiv);
 
 /* ---- configuration picker's perceptron, in double (mlp_oracle.c) ---- */
-#define ORACLE_MLP_IN 9
+#define ORACLE_MLP_IN 34
 #define ORACLE_MLP_HID 64
 #define ORACLE_MLP_OUT 20
 #define ORACLE_MLP_NPARAMS (ORACLE_MLP_IN * ORACLE_MLP_HID + ORACLE_MLP_HID + ORACLE_MLP_HID * ORACLE_MLP_HID + \

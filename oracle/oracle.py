@@ -241,7 +241,7 @@ def dgemm_blocked(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, nthreads
 
 
 # ------------------------------------------------------------------ picker perceptron (mlp_oracle.c)
-MLP_IN, MLP_HID, MLP_OUT = 9, 64, 20
+MLP_IN, MLP_HID, MLP_OUT = 34, 64, 20
 MLP_NPARAMS = MLP_IN * MLP_HID + MLP_HID + MLP_HID * MLP_HID + MLP_HID + MLP_HID * MLP_OUT + MLP_OUT
 
 

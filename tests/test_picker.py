@@ -19,20 +19,22 @@ from gemm_avx512f_amd import tuning
 from oracle import oracle as O
 
 P = O.MLP_NPARAMS
-OFF_W1, OFF_B1 = 0, 9 * 64
+NIN = O.MLP_IN
+OFF_W1, OFF_B1 = 0, NIN * 64
 OFF_W2, OFF_B2 = OFF_B1 + 64, OFF_B1 + 64 + 64 * 64
 OFF_W3 = OFF_B2 + 64
 OFF_B3 = OFF_W3 + 64 * 20
 
 
 def test_layout_size():
-    assert P == 6100 == tuning.MLP_NPARAMS == OFF_B3 + 20
+    assert NIN == 34 == tuning.MLP_IN
+    assert P == 7700 == tuning.MLP_NPARAMS == OFF_B3 + 20
 
 
 def test_forward_hand_computed():
     p = np.zeros(P)
     p[OFF_B3:OFF_B3 + 20] = np.arange(20)            # output = b3 when everything else is 0
-    x = np.random.default_rng(0).uniform(-1, 1, (3, 9))
+    x = np.random.default_rng(0).uniform(-1, 1, (3, NIN))
     assert np.array_equal(O.mlp_forward(p, x), np.tile(np.arange(20.0), (3, 1)))
     # identity paths: h1[j] = relu(x[j]) (j < 9), h2 = h1, y[0] = sum_j h2[j], y[1] = -2 * h2[0]
     p = np.zeros(P)
@@ -43,7 +45,8 @@ def test_forward_hand_computed():
     for j in range(9):
         p[OFF_W3 + j * 20 + 0] = 1.0
     p[OFF_W3 + 0 * 20 + 1] = -2.0
-    x = np.array([[0.5, -1.0, 2.0, 0, 0, 0, 0, 0, 0.25]])
+    x = np.zeros((1, NIN))
+    x[0, [0, 1, 2, 8]] = [0.5, -1.0, 2.0, 0.25]
     y = O.mlp_forward(p, x)
     assert y[0, 0] == 0.5 + 2.0 + 0.25          # the negative input is cut by the ReLU
     assert y[0, 1] == -1.0
@@ -53,7 +56,7 @@ def test_forward_hand_computed():
 def _random_case(seed, batch=5):
     rng = np.random.default_rng(seed)
     p = rng.normal(0, 0.3, P)
-    x = rng.uniform(-1, 1, (batch, 9))
+    x = rng.uniform(-1, 1, (batch, NIN))
     t = rng.uniform(0, 3, (batch, 20))
     mask = (rng.uniform(size=(batch, 20)) < 0.8).astype(np.float64)
     return p, x, t, mask
@@ -100,12 +103,65 @@ def test_adam_first_step_closed_form():
 
 
 def test_features_formula_points():
-    s = np.zeros(2, dtype=tuning.SHAPE_DTYPE)
+    s = np.zeros(4, dtype=tuning.SHAPE_DTYPE)
     s[0] = (1, 1, 0, 512, 16384, 16, 65)
     s[1] = (2, 0, 1, 16, 512, 512, 0)
+    s[2] = (0, 0, 0, 2048, 2048, 2048, 0)
+    s[3] = (0, 0, 0, 2185, 2185, 2048, 0)
     f = tuning.features(s)
-    assert np.allclose(f[0], [0, 1, -1, 1, 0, 0, 1, 0, 1 / 64], atol=1e-7)
-    assert np.allclose(f[1], [-1, 0, 0, 0, 1, 0, 0, 1, 0], atol=1e-7)
+    assert f.shape == (4, 34)
+    assert np.allclose(f[0, :9], [0, 1, -1, 1, 0, 0, 1, 0, 1 / 64], atol=1e-7)
+    assert np.allclose(f[1, :9], [-1, 0, 0, 0, 1, 0, 0, 1, 0], atol=1e-7)
+    # FP64, lda = K + 65 = 81 is odd: scalar loads.  Every tile grid is a
+    # whole number of 256-CU waves and K = 16 is one partition at any split.
+    assert f[0, 9] == 0 and np.all(f[0, 10:] == 1)
+    # CGEMM3M products read packed planes (M = 16, K = 512: vector path);
+    # 128x128 / 256x128 tiles: 4 of them, x K partitions 1..16 (K = 512 gives
+    # 512 / 32 slabs, so every split is honoured); 64x64 / 128x64: 8 tiles
+    assert f[1, 9] == 1
+    few = np.array([4, 8, 16, 32, 64]) / 256
+    assert np.allclose(f[1, 10:30], np.concatenate([few, 2 * few, 2 * few, few]), atol=1e-7)
+    assert np.allclose(f[1, 30:], [16 / 128, 16 / 64, 16 / 128, 16 / 256], atol=1e-7)
+    # 2048^2 FP32 fills the chip with 128x128 tiles exactly (256 of them);
+    # 2185^2 needs 18 x 18 = 324, a second wave 68/256 full
+    assert f[2, 10] == 1 and np.isclose(f[3, 10], 324 / 512)
+    assert f[2, 30] == 1 and np.isclose(f[3, 30], 2185 ** 2 / 2304 ** 2)
+    assert np.isclose(f[3, 11], 648 / 768)             # split 2: 648 workgroups, three waves
+
+
+def test_k_partitions_and_vec_path():
+    # each partition is a whole number of 32-deep slabs
+    assert list(tuning.k_partitions(np.array([16, 100, 512, 1000]), 4)) == [1, 4, 4, 4]
+    assert list(tuning.k_partitions(np.array([64, 65, 96]), 16)) == [2, 3, 3]
+    s = np.zeros(4, dtype=tuning.SHAPE_DTYPE)
+    s[0] = (0, 0, 0, 100, 10, 64, 0)     # lda 100, ldb 64: both multiples of 4 floats
+    s[1] = (0, 0, 0, 100, 10, 64, 2)     # lda 102
+    s[2] = (1, 1, 1, 7, 9, 64, 0)        # FP64 lda = K = 64, ldb = N = 9 (odd)
+    s[3] = (2, 1, 0, 8, 3, 12, 5)        # CGEMM3M: planes M = 8, K = 12
+    assert list(tuning.vec_path(s)) == [True, False, False, True]
+
+
+def test_library_features_equal_python_features():
+    # the library's mlp_features (host code) against tuning.features on both
+    # samplers' shapes, at the MI355X CU count and at another
+    s = np.concatenate([tuning.sample_shapes(5, 3000), tuning.sample_shapes_aligned(6, 3000)])
+    for cus in (256, 304):
+        lib = tuning.library_features(s, cus=cus)
+        py = tuning.features(s, cus=cus)
+        assert np.array_equal(lib, py), np.argwhere(lib != py)[:5]
+
+
+def test_sample_shapes_aligned():
+    s = tuning.sample_shapes_aligned(3, 6000)
+    assert np.array_equal(s, tuning.sample_shapes_aligned(3, 6000))
+    for key in ("m", "n", "k"):
+        v = s[key]
+        assert v.min() >= 64 and v.max() <= 16384 and np.all(v % 64 == 0)
+        # half the draws are powers of two; a quarter of the multiples of 64
+        # (those <= 128 mostly) are too
+        assert abs(np.mean((v & (v - 1)) == 0) - 0.63) < 0.03
+    assert np.all(np.bincount(s["op"]) == 2000)
+    assert abs(np.mean(s["pad"] == 0) - 0.75) < 0.03
 
 
 def test_sample_shapes_distribution():
@@ -160,6 +216,25 @@ needs_data = pytest.mark.skipif(not (os.path.exists(HELD) and os.path.exists(WEI
 @needs_data
 def test_compiled_in_weights_equal_trained_weights():
     assert np.array_equal(_inc_params(), np.load(WEIGHTS)["params"])
+
+
+ALIGNED = os.path.join(DATA, "sweep_aligned_heldout.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(ALIGNED), reason="aligned held-out sweep not generated yet")
+def test_picker_regret_on_aligned_shapes():
+    # power-of-two and multiple-of-64 sizes, where whole waves of tiles decide
+    # the best configuration: 1 000 held-out shapes (seed 4)
+    d = np.load(ALIGNED, allow_pickle=False)
+    shapes, ms = d["shapes"], d["ms"]
+    y = O.mlp_forward(np.load(WEIGHTS)["params"].astype(np.float64), tuning.features(shapes).astype(np.float64))
+    choice = tuning.choose(y, tuning.valid_mask(shapes["op"], shapes["m"], shapes["n"], shapes["k"]))
+    r_mlp = tuning.regret(ms, choice)
+    r_heur = tuning.regret(ms, np.full(len(ms), 20))
+    assert len(ms) >= 1000
+    assert np.median(r_mlp) <= 0.01 and np.mean(r_mlp) <= 0.02 and np.percentile(r_mlp, 90) <= 0.05, (
+        np.median(r_mlp), np.mean(r_mlp), np.percentile(r_mlp, 90))
+    assert np.mean(r_mlp) < 0.25 * np.mean(r_heur)
 
 
 @needs_data

@@ -39,7 +39,7 @@ def case(seed, batch):
     rng = np.random.default_rng(seed)
     p = tuning.init_params(seed).astype(np.float64)
     p[O.MLP_NPARAMS - 20:] = rng.normal(0, 0.1, 20)
-    x = rng.uniform(-1, 1, (batch, 9))
+    x = rng.uniform(-1, 1, (batch, O.MLP_IN))
     t = rng.uniform(0, 3, (batch, 20))
     mask = (rng.uniform(size=(batch, 20)) < 0.8).astype(np.float64)
     return p, x, t, mask
@@ -157,9 +157,13 @@ needs_data = pytest.mark.skipif(not (os.path.exists(HELD) and os.path.exists(WEI
                                 reason="sweep data / trained weights not generated yet")
 
 
+ALIGNED = os.path.join(tuning.DATA, "sweep_aligned_heldout.npz")
+
+
 @needs_data
-def test_hip_inference_matches_oracle_picks(gpu_lib):
-    d = np.load(HELD, allow_pickle=False)
+@pytest.mark.parametrize("data", [HELD, ALIGNED], ids=["heldout", "aligned"])
+def test_hip_inference_matches_oracle_picks(gpu_lib, data):
+    d = np.load(data, allow_pickle=False)
     shapes = d["shapes"]
     params = np.load(WEIGHTS)["params"]
     tuning.picker_load(params)
