@@ -55,6 +55,9 @@ def _time(fn, stream, steps, warmup):
     return ms, time.perf_counter() - t0
 
 
+SAMPLES = 1024   # sampled entries of C per error figure
+
+
 def _sampled_error(a, b, c, rows, cols):
     """Max and RMS relative error of C at sampled entries against long-double
     dot products of the same (downloaded) rows of A and columns of B."""
@@ -86,7 +89,7 @@ def gpu_leg(size, steps, warmup, variants=(("crt", 16), ("slices", 8)), seed=0):
     ms, wall = _time(lambda: blas.dgemm("N", "N", m, n, k, 1.0, A, B, 0.0, C, stream=stream), stream, steps, warmup)
     out["native"] = {"ms": ms, "wall": wall}
     rng = np.random.default_rng(12345)
-    rows, cols = rng.integers(0, m, 256), rng.integers(0, n, 256)
+    rows, cols = rng.integers(0, m, SAMPLES), rng.integers(0, n, SAMPLES)
     a, b = A.to_host(), B.to_host()
     c_nat = C.to_host()
     if not np.isfinite(c_nat).all():
@@ -126,8 +129,8 @@ def accuracy_leg(size, ranges=(0, 8, 32), seed=7):
     stream = Stream()
     A, B, C = DeviceMatrix(m, k, np.float64), DeviceMatrix(k, n, np.float64), DeviceMatrix(m, n, np.float64)
     rng = np.random.default_rng(999)
-    rows, cols = rng.integers(0, m, 256), rng.integers(0, n, 256)
-    out = {"workload": "DGEMM %d^3 NN, entries u*2^e, e uniform in [-r, r]; 256 sampled entries" % size}
+    rows, cols = rng.integers(0, m, SAMPLES), rng.integers(0, n, SAMPLES)
+    out = {"workload": "DGEMM %d^3 NN, entries u*2^e, e uniform in [-r, r]; %d sampled entries" % (size, SAMPLES)}
     runs = [("native_fp64", lambda: blas.dgemm("N", "N", m, n, k, 1.0, A, B, 0.0, C, stream=stream))]
     for nm in (16, 15, 14, 13, 12):
         runs.append(("crt_%d_moduli" % nm, lambda nm=nm: blas.dgemm_ozaki_crt("N", "N", m, n, k, 1.0, A, B, 0.0, C,
@@ -162,7 +165,7 @@ def sgemm_leg(size, steps, warmup, seed=0):
     fill_uniform(B, seed, d, stream=stream)
     a, b = A.to_host().astype(np.float64), B.to_host().astype(np.float64)
     rng = np.random.default_rng(54321)
-    rows, cols = rng.integers(0, m, 256), rng.integers(0, n, 256)
+    rows, cols = rng.integers(0, m, SAMPLES), rng.integers(0, n, SAMPLES)
     flops = 2.0 * m * n * k
     out = {"workload": "SGEMM %d^3 NN, alpha=1, beta=0, uniform[-1,1) FP32" % size}
     runs = (("native_fp32", lambda: blas.sgemm("N", "N", m, n, k, 1.0, A, B, 0.0, C, stream=stream), 157.3),
@@ -194,7 +197,7 @@ def configs_leg(steps=20, warmup=3, seed=0):
     d = fill_uniform(A, seed, 0, stream=stream)
     fill_uniform(B, seed, d, stream=stream)
     ms, _ = _time(lambda: blas.sgemm("N", "N", n, n, n, 1.0, A, B, 0.0, C, stream=stream), stream, steps, warmup)
-    rows, cols = rng.integers(0, n, 256), rng.integers(0, n, 256)
+    rows, cols = rng.integers(0, n, SAMPLES), rng.integers(0, n, SAMPLES)
     err = _sampled_error(A.to_host().astype(np.float64), B.to_host().astype(np.float64),
                          C.to_host().astype(np.float64), rows, cols)
     out["sgemm_1024_nn"] = {"tflops": round(2.0 * n ** 3 * steps / (ms / 1e3) / 1e12, 2),
@@ -205,7 +208,7 @@ def configs_leg(steps=20, warmup=3, seed=0):
     d = fill_uniform(B, seed, d, stream=stream)
     fill_uniform(C, seed, d, stream=stream)
     a, b, c0 = A.to_host(), B.to_host(), C.to_host()
-    rows, cols = rng.integers(0, n, 256), rng.integers(0, n, 256)
+    rows, cols = rng.integers(0, n, SAMPLES), rng.integers(0, n, SAMPLES)
     for ta in ("N", "T"):
         C.buf.upload(np.asfortranarray(c0))
         blas.dgemm(ta, "N", n, n, n, 1.5, A, B, 0.5, C, stream=stream)   # one checked call from the known C
@@ -463,9 +466,11 @@ def main():
     ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-emulated", action="store_true", help="skip the emulated (int8) DGEMM legs")
-    ap.add_argument("--nslices", type=int, default=8, help="slices per operand of the slice-form emulated DGEMM")
-    ap.add_argument("--nmoduli", type=int, default=16,
-                    help="moduli of the modular-form emulated DGEMM (timed at this count and one fewer)")
+    ap.add_argument("--nslices", type=int, default=0,
+                    help="slices per operand of the slice-form emulated DGEMM (0: the library's tuned count for K)")
+    ap.add_argument("--nmoduli", type=int, default=0,
+                    help="moduli of the modular-form emulated DGEMM, timed at this count and one fewer "
+                         "(0: the library's tuned count for K)")
     ap.add_argument("--cpu-size", type=int, default=None, help="CPU leg problem size (default: --size)")
     ap.add_argument("--no-grouped", action="store_true", help="skip the grouped-GEMM / zgemm3m batch leg")
     ap.add_argument("--no-sgemm", action="store_true", help="skip the SGEMM / bf16x split leg")
@@ -495,7 +500,13 @@ def main():
         _lib.check(_lib.lib().gemm_amd_memcpy(dst.ptr, src.ptr, nb, 3), "calibration copy")
         synchronize()
         del src, dst
-    variants = () if args.no_emulated else (("crt", args.nmoduli), ("crt", args.nmoduli - 1), ("slices", args.nslices))
+    nm_default, ns_default = 0, 0
+    if not args.no_emulated:
+        from gemm_avx512f_amd.tuning import emulation_counts
+        nm_default, ns_default = emulation_counts(size)   # what a call with 0 runs at this K
+    nmoduli = args.nmoduli or nm_default
+    nslices = args.nslices or ns_default
+    variants = () if args.no_emulated else (("crt", nmoduli), ("crt", nmoduli - 1), ("slices", nslices))
     _progress("native and emulated DGEMM %d^3" % size)
     legs = gpu_leg(size, args.steps, args.warmup, variants=variants)
     ms, wall = legs["native"]["ms"], legs["native"]["wall"]
@@ -528,7 +539,7 @@ def main():
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(flops / (ms_step / 1e3) / 1e12 / PEAK_FP64_TFLOPS, 4), "traffic": traffic},
         "native_err": {"max_rel": legs["native"]["err"][0], "rms_rel": legs["native"]["err"][1],
-                       "sample": "256 entries vs long-double dot products"},
+                       "sample": "%d entries vs long-double dot products" % SAMPLES},
         "wall_s_timed_region": round(wall, 3),
     }
     from gemm_avx512f_amd import _lib
@@ -563,10 +574,17 @@ def main():
         variants[-1]["fp64_level_at"] = at
         variants[-1]["fp64_level"] = all(at.values())
         variants[-1]["faster_than_native"] = bool(em_tf > tflops)
+        variants[-1]["library_default"] = bool(e["count"] == (nm_default if e["kind"] == "crt" else ns_default))
     if variants:
-        # headline emulated figure: the fastest variant at FP64 level on every input range
+        # headline emulated figure: the modular form at the library's own count
+        # for this K (what a call with nmoduli = 0 runs) when it is at FP64
+        # level on every input range; else the fastest variant that is
         ok = [v for v in variants if v["fp64_level"]]
-        result["emulated"] = max(ok, key=lambda v: v["tflops_fp64_equiv"]) if ok else None
+        default = [v for v, e in zip(variants, legs["emulated"]) if e["kind"] == "crt" and v["library_default"]]
+        if default and default[0]["fp64_level"]:
+            result["emulated"] = default[0]
+        else:
+            result["emulated"] = max(ok, key=lambda v: v["tflops_fp64_equiv"]) if ok else None
         result["emulated_variants"] = variants
     if accuracy is not None:
         result["accuracy_vs_range"] = accuracy

@@ -351,21 +351,25 @@ struct WaveF32 {
                 for (int sm = 0; sm < SM; ++sm)
 #pragma unroll
                     for (int sn = 0; sn < SN; ++sn)
-                        acc[sm][sn] = mfma_f32_32x32x2(av[sm][e], bv[sn][e], acc[sm][sn]);
+                        acc[sm][sn] = mfma_f32_32x32x2(bv[sn][e], av[sm][e], acc[sm][sn]);
             __builtin_amdgcn_sched_barrier(0);
         }
     }
 
+    // B's fragment is the matrix instruction's first operand, so each
+    // accumulator block is the transposed tile: lane l holds row l % 32 of C,
+    // and one element index across the wave is 32 consecutive rows of two
+    // columns (whole 128-byte runs of column-major C for the epilogue).
     template <typename F>
     __device__ __forceinline__ void for_each(int lane, int wm0, int wn0, F &&visit) const {
-        const int j = lane & 31, g = lane >> 5;
+        const int i = lane & 31, g = lane >> 5;
 #pragma unroll
         for (int sm = 0; sm < SM; ++sm)
 #pragma unroll
             for (int sn = 0; sn < SN; ++sn)
 #pragma unroll
                 for (int e = 0; e < 16; ++e)
-                    visit(wm0 + sm * 32 + (e >> 2) * 8 + g * 4 + (e & 3), wn0 + sn * 32 + j, acc[sm][sn][e]);
+                    visit(wm0 + sm * 32 + i, wn0 + sn * 32 + (e >> 2) * 8 + g * 4 + (e & 3), acc[sm][sn][e]);
     }
 };
 
@@ -429,32 +433,46 @@ struct WaveBF16 {
                 for (int e = 0; e < 16; ++e) acc[sm][sn][e] = 0.0f;
     }
 
+    // The fragments of k-step s + 1 are read while the matrix instructions of
+    // step s run (the sched barriers keep the scheduler from sinking each read
+    // next to its use, which would leave the matrix pipe waiting on LDS).
     __device__ __forceinline__ void compute(const uint16_t *As, const uint16_t *Bs) {
+        bf8_t av[2][SM], bv[2][SN];
+        auto load = [&](int s, bf8_t *a, bf8_t *b) {
+#pragma unroll
+            for (int sm = 0; sm < SM; ++sm) a[sm] = frag<AKP>(As, aoff[sm], s);
+#pragma unroll
+            for (int sn = 0; sn < SN; ++sn) b[sn] = frag<BKP>(Bs, boff[sn], s);
+        };
+        load(0, av[0], bv[0]);
 #pragma unroll
         for (int s = 0; s < BK / 16; ++s) {
-            bf8_t av[SM], bv[SN];
-#pragma unroll
-            for (int sm = 0; sm < SM; ++sm) av[sm] = frag<AKP>(As, aoff[sm], s);
-#pragma unroll
-            for (int sn = 0; sn < SN; ++sn) bv[sn] = frag<BKP>(Bs, boff[sn], s);
+            if (s + 1 < BK / 16) load(s + 1, av[(s + 1) & 1], bv[(s + 1) & 1]);
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int sm = 0; sm < SM; ++sm)
 #pragma unroll
                 for (int sn = 0; sn < SN; ++sn)
-                    acc[sm][sn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[sm], bv[sn], acc[sm][sn], 0, 0, 0);
+                    acc[sm][sn] =
+                        __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv[s & 1][sn], av[s & 1][sm], acc[sm][sn], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 
+    // The matrix instructions take B's fragment as their first operand, so each
+    // accumulator block is the transposed tile: lane l holds row l % 32 of C
+    // (and 16 of its columns), and one element index across the wave is 32
+    // consecutive rows of two columns, whole 128-byte runs of column-major C.
     template <typename F>
     __device__ __forceinline__ void for_each(int lane, int wm0, int wn0, F &&visit) const {
-        const int j = lane & 31, g = lane >> 5;
+        const int i = lane & 31, g = lane >> 5;
 #pragma unroll
         for (int sm = 0; sm < SM; ++sm)
 #pragma unroll
             for (int sn = 0; sn < SN; ++sn)
 #pragma unroll
                 for (int e = 0; e < 16; ++e)
-                    visit(wm0 + sm * 32 + (e >> 2) * 8 + g * 4 + (e & 3), wn0 + sn * 32 + j, acc[sm][sn][e]);
+                    visit(wm0 + sm * 32 + i, wn0 + sn * 32 + (e >> 2) * 8 + g * 4 + (e & 3), acc[sm][sn][e]);
     }
 };
 
