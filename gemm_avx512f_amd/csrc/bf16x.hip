@@ -176,21 +176,24 @@ __global__ __launch_bounds__(WM *WN * 64) void bf16x_gemm_kernel(BxParams p) {
             for (int a = 0; a < TM; ++a)
 #pragma unroll
                 for (int b = 0; b < TN; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[ks & 1][a], bv[ks & 1][b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv[ks & 1][b], av[ks & 1][a], acc[a][b], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
         if (st + 1 < total) sstore(nxt);
         __syncthreads();
     }
 
+    // B's fragment is the matrix instructions' first operand, so lane q holds
+    // row q of each 32x32 block: one element index across the wave writes 32
+    // consecutive rows of two columns (whole 128-byte runs of column-major C).
 #pragma unroll
     for (int a = 0; a < TM; ++a)
 #pragma unroll
         for (int b = 0; b < TN; ++b)
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
-                const int i = m0 + wm0 + 32 * a + (e >> 2) * 8 + g * 4 + (e & 3);
-                const int j = n0 + wn0 + 32 * b + q;
+                const int i = m0 + wm0 + 32 * a + q;
+                const int j = n0 + wn0 + 32 * b + (e >> 2) * 8 + g * 4 + (e & 3);
                 if (i < p.M && j < p.N) {
                     float *dst = p.C + i + (int64_t)j * p.ldc;
                     const float v = p.alpha * acc[a][b][e];
