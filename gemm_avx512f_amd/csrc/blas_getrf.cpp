@@ -20,11 +20,6 @@ int getrf_check(int m, int n, int lda) {
 }
 
 int getrf_run(int m, int n, double *a, int lda, int *ipiv, int *info, hipStream_t s) {
-    if (m > GETRF_MAX_ROWS) {
-        set_error("DGETRF: m = " + std::to_string(m) + " exceeds the supported maximum of " +
-                  std::to_string(GETRF_MAX_ROWS) + " rows");
-        return -(int)hipErrorInvalidValue;
-    }
     const int mn = m < n ? m : n;
     int st = 0;
     WsLease lease(s);
@@ -61,15 +56,6 @@ void dgetrf_(const int *pm, const int *pn, double *a, const int *plda, int *ipiv
     }
     *info = 0;
     if (m == 0 || n == 0) return;
-    if (m > GETRF_MAX_ROWS) {
-        // a legal LAPACK argument this library does not support: reported as
-        // an illegal first argument, nothing factored
-        *info = -1;
-        fprintf(stderr, " ** On entry to DGETRF parameter number  1 had an unsupported value (m = %d > %d)\n", m,
-                GETRF_MAX_ROWS);
-        set_error("DGETRF: m exceeds the supported maximum of 16384 rows");
-        return;
-    }
     // every failure below leaves info = GEMM_AMD_INFO_RUNTIME_ERROR (the
     // matrix and ipiv are then not a factorisation)
     if (init_status()) {

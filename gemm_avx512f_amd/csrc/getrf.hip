@@ -6,7 +6,8 @@
 // columns run in the DTRSM and FP64 matrix-core GEMM kernels of this library.
 // A leaf is a panel of up to 32 columns and every row below its diagonal,
 // factored by one 1024-thread workgroup with the panel held in registers
-// (rows strided over threads).  After each leaf a chip-wide kernel applies
+// (rows strided over threads); panels of more than 16384 rows stay in global
+// memory instead (getrf_leaf_global_kernel), so m is not limited.  After each leaf a chip-wide kernel applies
 // the leaf's row interchanges to every other column of the matrix, so the
 // state after every leaf is that of LAPACK's unblocked dgetf2 at the same
 // column and no deferred row-swap passes are needed.
@@ -167,6 +168,104 @@ __global__ __launch_bounds__(LU_THREADS) void getrf_leaf_kernel(double *__restri
     }
 }
 
+// Panels taller than the register leaves hold (more than 16384 rows below the
+// diagonal): the same column steps with the panel left in global memory.  Row
+// r of the panel belongs to thread (r - c0) mod 1024 for the whole leaf, so
+// every panel entry is read and written by one thread only (its own program
+// order keeps it coherent); what the other threads need of a column step --
+// row cj and the pivot row -- passes through LDS, double buffered by column
+// parity as in the register leaf.  ipiv and info as getrf_leaf_kernel.
+template <int W>
+__global__ __launch_bounds__(1024) void getrf_leaf_global_kernel(double *__restrict__ A, int64_t lda, int m, int c0,
+                                                                 int w, int *__restrict__ ipiv,
+                                                                 int *__restrict__ info) {
+    constexpr int NT = 1024, NW = NT / 64;
+    __shared__ double s_mag[2][NW];
+    __shared__ int s_row[2][NW];
+    __shared__ double s_prow[2][W];   // the pivot row's panel entries (become row c0+j)
+    __shared__ double s_crow[2][W];   // row c0+j's panel entries (move to the pivot row)
+    __shared__ int s_ipiv[W];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int zero_col = -1;
+    for (int j = 0; j < w; ++j) {
+        const int cj = c0 + j, b = j & 1;
+        const double *colj = A + (int64_t)cj * lda;
+        // 1. row cj (thread j owns it); pivot candidates over the thread's rows >= cj
+        if (tid == j)
+#pragma unroll
+            for (int k = 0; k < W; ++k) s_crow[b][k] = k < w ? A[cj + (int64_t)(c0 + k) * lda] : 0.0;
+        double best = -1.0;
+        int brow = 0x7fffffff;
+        for (int r = c0 + tid; r < m; r += NT) {
+            if (r < cj) continue;
+            const double mag = fabs(colj[r]);
+            if (mag > best) {   // rows ascend: a strict comparison keeps the first maximum
+                best = mag;
+                brow = r;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double om = __shfl_xor(best, off);
+            const int orow = __shfl_xor(brow, off);
+            if (pivot_better(om, orow, best, brow)) {
+                best = om;
+                brow = orow;
+            }
+        }
+        if (lane == 0) {
+            s_mag[b][wave] = best;
+            s_row[b][wave] = brow;
+        }
+        __syncthreads();
+        double bm = lane < NW ? s_mag[b][lane] : -1.0;
+        int br = lane < NW ? s_row[b][lane] : 0x7fffffff;
+#pragma unroll
+        for (int off = NW / 2; off > 0; off >>= 1) {
+            const double om = __shfl_xor(bm, off);
+            const int orow = __shfl_xor(br, off);
+            if (pivot_better(om, orow, bm, br)) {
+                bm = om;
+                br = orow;
+            }
+        }
+        br = __shfl(br, 0);
+        const int p = br >= m ? cj : br;   // no candidate compared larger (NaN column): keep the diagonal row
+        const int powner = (p - c0) % NT;
+        if (tid == 0) s_ipiv[j] = p + 1;
+        if (tid == powner)
+#pragma unroll
+            for (int k = 0; k < W; ++k) s_prow[b][k] = k < w ? A[p + (int64_t)(c0 + k) * lda] : 0.0;
+        __syncthreads();
+        // 2. swap rows cj and p (each by its owner), scale, rank-1 update of the later panel columns
+        const double piv = s_prow[b][j];
+        if (tid == 0 && piv == 0.0 && zero_col < 0) zero_col = cj;
+        if (p != cj) {
+            if (tid == j)
+                for (int k = 0; k < w; ++k) A[cj + (int64_t)(c0 + k) * lda] = s_prow[b][k];
+            if (tid == powner)
+                for (int k = 0; k < w; ++k) A[p + (int64_t)(c0 + k) * lda] = s_crow[b][k];
+        }
+        if (piv != 0.0) {
+            const bool recip = fabs(piv) >= DBL_MIN;
+            const double rpiv = 1.0 / piv;
+            for (int r = c0 + tid; r < m; r += NT) {
+                if (r <= cj) continue;
+                double *row = A + r;
+                const double l = recip ? row[(int64_t)cj * lda] * rpiv : row[(int64_t)cj * lda] / piv;
+                row[(int64_t)cj * lda] = l;
+                for (int k = j + 1; k < w; ++k) row[(int64_t)(c0 + k) * lda] = fma(-l, s_prow[b][k], row[(int64_t)(c0 + k) * lda]);
+            }
+        }
+    }
+    __syncthreads();   // s_ipiv is complete
+    if (tid < w) ipiv[c0 + tid] = s_ipiv[tid];
+    if (tid == 0) {
+        if (c0 == 0) *info = zero_col >= 0 ? zero_col + 1 : 0;
+        else if (zero_col >= 0 && *info == 0) *info = zero_col + 1;
+    }
+}
+
 // Applies the interchanges ipiv[c0 .. c0+w) (1-based rows, in order) to every
 // column in [0, ncols) outside [c0, c0+w).  Each workgroup first reduces the
 // w swaps to their net effect on the rows they touch (the value that ends at
@@ -267,9 +366,13 @@ __global__ __launch_bounds__(64) void getrf_trsm_small_kernel(const double *__re
         if (i < n1) col[i] = x[i];
 }
 
+// T > 0: the register leaf (T rows per thread); T == 0: the global-memory leaf.
 template <int T, int W>
 int leaf(double *A, int64_t lda, int m, int ncols, int c0, int w, int *ipiv, int *info, hipStream_t s) {
-    hipLaunchKernelGGL((getrf_leaf_kernel<T, W>), dim3(1), dim3(LU_THREADS), 0, s, A, lda, m, c0, w, ipiv, info);
+    if constexpr (T > 0)
+        hipLaunchKernelGGL((getrf_leaf_kernel<T, W>), dim3(1), dim3(LU_THREADS), 0, s, A, lda, m, c0, w, ipiv, info);
+    else
+        hipLaunchKernelGGL((getrf_leaf_global_kernel<W>), dim3(1), dim3(1024), 0, s, A, lda, m, c0, w, ipiv, info);
     if (w > 0 && ncols > w) {
         constexpr int cols_per_block = SWAP_THREADS / swap_lanes_per_column(W);
         const int grid = (ncols + cols_per_block - 1) / cols_per_block;
@@ -292,10 +395,11 @@ struct Lu {
 
     double *at(int r, int c) const { return A + r + (int64_t)c * lda; }
 
-    // widest leaf the register budget allows for a panel of `rows` rows
+    // widest leaf the register budget allows for a panel of `rows` rows; past
+    // GETRF_REGISTER_ROWS the panel stays in global memory, 8 columns a leaf
     static int leaf_width(int rows) {
         return rows <= LU_THREADS ? 32 : rows <= 2 * LU_THREADS ? 16 : rows <= 4 * LU_THREADS ? 8
-               : rows <= 8 * LU_THREADS ? 4 : 2;
+               : rows <= 8 * LU_THREADS ? 4 : rows <= GETRF_REGISTER_ROWS ? 2 : 8;
     }
 
     void run_leaf(int c0, int w) {
@@ -305,7 +409,8 @@ struct Lu {
         else if (rows <= 2 * LU_THREADS) err = leaf<2, 16>(A, lda, m, n, c0, w, ipiv, info, s);
         else if (rows <= 4 * LU_THREADS) err = leaf<4, 8>(A, lda, m, n, c0, w, ipiv, info, s);
         else if (rows <= 8 * LU_THREADS) err = leaf<8, 4>(A, lda, m, n, c0, w, ipiv, info, s);
-        else err = leaf<16, 2>(A, lda, m, n, c0, w, ipiv, info, s);
+        else if (rows <= GETRF_REGISTER_ROWS) err = leaf<16, 2>(A, lda, m, n, c0, w, ipiv, info, s);
+        else err = leaf<0, 8>(A, lda, m, n, c0, w, ipiv, info, s);
     }
 
     void gemm(int mm, int nn, int kk, const double *a, const double *b, double *c) {
@@ -355,16 +460,17 @@ struct Lu {
 
 size_t getrf_gemm_workspace_bytes(int m, int n) {
     // split-K partials of the widest Schur update, at most 8 partitions of an
-    // m x min(m, n)/2 block (the recursion never updates a wider one; the
-    // solve for the columns right of a wide matrix's square part takes
-    // split-K only where its partials fit, and runs unsplit otherwise)
+    // m x min(m, n)/2 block (the recursion never updates a wider one), capped
+    // at 512 MiB: a GEMM whose partials do not fit runs unsplit (large
+    // updates fill the chip without splitting K)
     const int mn = m < n ? m : n;
-    return (size_t)8 * (size_t)(m > 0 ? m : 1) * (size_t)((mn + 1) / 2 + 1) * sizeof(double);
+    const size_t want = (size_t)8 * (size_t)(m > 0 ? m : 1) * (size_t)((mn + 1) / 2 + 1) * sizeof(double);
+    const size_t cap = (size_t)512 << 20;
+    return want < cap ? want : cap;
 }
 
 int launch_dgetrf(int m, int n, double *A, int64_t lda, int *ipiv, int *info, double *inv_ws, void *gemm_ws,
                   size_t gemm_ws_bytes, int cus, hipStream_t s) {
-    if (m > GETRF_MAX_ROWS) return (int)hipErrorInvalidValue;
     Lu lu{A, lda, m, n, ipiv, info, inv_ws, gemm_ws, gemm_ws_bytes, cus, s};
     const int mn = m < n ? m : n;
     if (mn == 0) lu.run_leaf(0, 0);   // writes info = 0 and nothing else

@@ -70,8 +70,9 @@ void zgemm3m_(const char *transa, const char *transb, const int *m, const int *n
  * argument i, or i > 0 when U(i,i) is exactly zero (the factorisation is still
  * completed).  No counterpart in the reference, which has no LAPACK layer: it
  * is the first caller of the DTRSM and DGEMM paths above, which carry all but
- * the panel columns.  m <= 16384: a taller matrix gets info = -1 and is left
- * untouched.  A failure of the GPU runtime (initialisation, memory, a launch
+ * the panel columns.  Any m: panels of up to 16384 rows are factored in
+ * registers, taller ones in global memory.  A failure of the GPU runtime
+ * (initialisation, memory, a launch
  * or a copy) gives info = GEMM_AMD_INFO_RUNTIME_ERROR, and a and ipiv are then
  * not a factorisation. */
 #define GEMM_AMD_INFO_RUNTIME_ERROR (-1000)
@@ -151,7 +152,7 @@ int gemm_amd_dtrsm_kernel(int variant, long M, long N, double *sa, double *sb, d
 
 /* Device-pointer dgetrf_: a, ipiv and info in device memory, stream-ordered.
  * Returns 0, the index of an illegal argument (m, n, -, lda = 1, 2, -, 4), or a
- * negative HIP error (m > 16384 gives -hipErrorInvalidValue). */
+ * negative HIP error. */
 int gemm_amd_dgetrf(int m, int n, double *a, int lda, int *ipiv, int *info, void *stream);
 
 int gemm_amd_sgemm(char transa, char transb, int m, int n, int k, float alpha,

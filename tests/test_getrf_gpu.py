@@ -1,8 +1,9 @@
 """DGETRF on the GPU through the C ABI against the LU oracle (LAPACK dgetf2
 semantics): identical pivots, factors within rounding of the oracle's,
-backward error of P A = L U, every leaf-panel shape (1024 to 16384 rows),
-wide and tall matrices, padded lda, zero-pivot info, an exactly representable
-factorisation (bitwise), the host-pointer dgetrf_, and argument errors."""
+backward error of P A = L U, every leaf-panel shape (1024 to 16384 rows in
+registers, taller panels in global memory), wide and tall matrices, padded
+lda, zero-pivot info, an exactly representable factorisation (bitwise), the
+host-pointer dgetrf_, and argument errors."""
 import numpy as np
 import pytest
 
@@ -54,6 +55,8 @@ def backward_error(a0, lu, ipiv, m, n):
     (3000, 700, 0, 12),    # 4 rows per thread, 8-column leaves
     (5000, 300, 0, 13),    # 8 rows per thread, 4-column leaves
     (9000, 64, 0, 14),     # 16 rows per thread, 2-column leaves
+    (20000, 40, 3, 15),    # panels of more than 16384 rows: global-memory leaves of 8 columns
+    (40000, 20, 0, 16),
 ])
 def test_matches_oracle(gpu_lib, m, n, pad, seed):
     print(f"dgetrf m={m} n={n} pad={pad} seed={seed}", flush=True)
@@ -145,9 +148,6 @@ def test_argument_errors(gpu_lib, capfd):
     dA = DeviceMatrix(4, 4, np.float64)
     dpiv, dinfo = DeviceBuffer(16), DeviceBuffer(4)
     assert _lib.lib().gemm_amd_dgetrf(4, 4, dA.ptr, 3, dpiv.ptr, dinfo.ptr, None) == 4
-    # taller than the register-resident panels cover: a clear error, nothing launched
-    rc = _lib.lib().gemm_amd_dgetrf(16385, 1, dA.ptr, 16385, dpiv.ptr, dinfo.ptr, None)
-    assert rc < 0 and "16384" in _lib.last_error()
     # m = 0 / n = 0: info is still written
     dinfo.upload(np.array([123], np.int32))
     assert _lib.lib().gemm_amd_dgetrf(0, 4, dA.ptr, 1, dpiv.ptr, dinfo.ptr, None) == 0
