@@ -317,7 +317,19 @@ def grouped_leg(replays=20):
             c = DeviceMatrix(m, n, np.float32)
             probs.append(dict(transa=ta, transb=tb, m=m, n=n, k=k, alpha=1.0, beta=0.0, a=a, b=b, c=c))
             flops += 2.0 * m * n * k
-        g = GroupedGemm(dtype, probs)
+        # load balance of the persistent launch under both schedules: per
+        # workgroup start/end times of one (uncaptured) launch
+        balance = {}
+        for sched, dyn in (("static", False), ("dynamic", True)):
+            gs = GroupedGemm(dtype, probs, dynamic=dyn)
+            gs.timeline(stream)   # warm-up
+            st, en, _ = gs.timeline(stream)
+            span = float(en.max())
+            balance[sched] = {"makespan_us": round(span, 1),
+                              "busy_pct": round(100.0 * float((en - st).sum()) / (len(st) * span), 1),
+                              "tail_us": round(span - float(np.median(en)), 1)}
+            gs.destroy()
+        g = GroupedGemm(dtype, probs)   # the library's default schedule (dynamic)
         g.capture(stream)
         for _ in range(3):
             g.replay(stream)
@@ -329,7 +341,10 @@ def grouped_leg(replays=20):
         e1.record(stream)
         ms = e0.elapsed_ms(e1) / replays
         out[name] = {"problems": len(probs), "gflop": round(flops / 1e9, 2), "ms_per_replay": round(ms, 4),
-                     "tflops": round(flops / (ms / 1e3) / 1e12, 2), "tiles": g.tiles}
+                     "tflops": round(flops / (ms / 1e3) / 1e12, 2), "tiles": g.tiles, "workgroups": g.grid,
+                     "schedule": "dynamic", "load_balance": balance,
+                     "load_balance_note": "busy_pct = sum over workgroups of (end - start) / (workgroups x "
+                                          "makespan); tail_us = makespan - median workgroup end"}
         g.destroy()
         del probs
     rng = np.random.default_rng(128)

@@ -89,6 +89,8 @@ SIGNATURES = {
     "gemm_amd_grouped_capture": (c_int, [c_vp, c_vp]),
     "gemm_amd_grouped_replay": (c_int, [c_vp, c_vp]),
     "gemm_amd_grouped_tiles": (ctypes.c_int64, [c_vp]),
+    "gemm_amd_grouped_set_schedule": (c_int, [c_vp, c_int, c_vp]),
+    "gemm_amd_grouped_grid": (c_int, [c_vp]),
     "gemm_amd_grouped_destroy": (c_int, [c_vp]),
     "gemm_amd_zgemm3m_batched": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                          c_vp, c_vp, c_vp]),

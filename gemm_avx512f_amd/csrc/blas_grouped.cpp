@@ -77,6 +77,17 @@ int gemm_amd_grouped_replay(void *plan, void *stream) {
 
 int64_t gemm_amd_grouped_tiles(void *plan) { return plan ? grouped_plan_tiles(plan) : -1; }
 
+int gemm_amd_grouped_set_schedule(void *plan, int dynamic, unsigned long long *timeline) {
+    if (!plan) return -(int)hipErrorInvalidValue;
+    if (grouped_plan_configure(plan, dynamic, timeline)) {
+        set_error("gemm_amd_grouped_set_schedule: the plan is already captured (its launch keeps its arguments)");
+        return -(int)hipErrorInvalidValue;
+    }
+    return 0;
+}
+
+int gemm_amd_grouped_grid(void *plan) { return plan ? grouped_plan_grid(plan) : -1; }
+
 int gemm_amd_grouped_destroy(void *plan) {
     if (!plan) return 0;
     hipError_t e = hipDeviceSynchronize();   // the descriptor table may still be in use

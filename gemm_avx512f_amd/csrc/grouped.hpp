@@ -31,5 +31,10 @@ int grouped_plan_capture(void *plan, hipStream_t s);   // capture that launch in
 int grouped_plan_replay(void *plan, hipStream_t s);    // hipGraphLaunch of the captured graph
 int grouped_plan_destroy(void *plan);
 int64_t grouped_plan_tiles(void *plan);
+// Before capture: dynamic != 0 claims tiles from an atomic counter instead of
+// the static stride; timeline (device, 3 x grid words, or null) receives each
+// workgroup's start and end time (100 MHz counter) and tile count.
+int grouped_plan_configure(void *plan, int dynamic, unsigned long long *timeline);
+int grouped_plan_grid(void *plan);   // workgroups of the persistent launch
 
 }  // namespace gemm_amd

@@ -31,9 +31,11 @@ class GroupedGemm:
 
     dtype: GROUPED_F32, GROUPED_BF16 (operands hold bf16 bit patterns as
     uint16, C is FP32) or GROUPED_F64.  large_min: smallest M and N of a
-    problem on the large tile class (None: the library's default, 128)."""
+    problem on the large tile class (None: the library's default for the
+    dtype: bf16 96, FP64 128, FP32 none).  dynamic: workgroups claim tiles
+    from an atomic counter (the default) or take the static stride."""
 
-    def __init__(self, dtype, problems, large_min=None):
+    def __init__(self, dtype, problems, large_min=None, dynamic=True):
         self.dtype = dtype
         self.problems = list(problems)
         for p in self.problems:
@@ -58,10 +60,36 @@ class GroupedGemm:
             self.handle = L.gemm_amd_grouped_create_ex(dtype, cnt, *self._keep, int(large_min))
         if not self.handle:
             raise _lib.GemmAmdError("grouped_create: " + _lib.last_error())
+        self.dynamic = bool(dynamic)
+        _lib.check(L.gemm_amd_grouped_set_schedule(self.handle, int(self.dynamic), None), "grouped_set_schedule")
 
     @property
     def tiles(self):
         return int(_lib.lib().gemm_amd_grouped_tiles(self.handle))
+
+    @property
+    def grid(self):
+        return int(_lib.lib().gemm_amd_grouped_grid(self.handle))
+
+    def timeline(self, stream=None):
+        """Runs the plan once (not captured) with per-workgroup timing and
+        returns (start, end, tiles) arrays: start and end in microseconds from
+        the first workgroup's start (100 MHz real-time counter), tiles
+        processed per workgroup.  Call before capture()."""
+        from .device import DeviceBuffer, synchronize
+        L = _lib.lib()
+        g = self.grid
+        buf = DeviceBuffer(8 * 3 * g)
+        _lib.check(L.gemm_amd_grouped_set_schedule(self.handle, int(self.dynamic), ctypes.c_void_p(buf.ptr)),
+                   "grouped_set_schedule")
+        try:
+            self.run(stream)
+            synchronize()
+            out = buf.download(np.zeros(3 * g, dtype=np.uint64)).reshape(g, 3)
+        finally:
+            _lib.check(L.gemm_amd_grouped_set_schedule(self.handle, int(self.dynamic), None), "grouped_set_schedule")
+        t0 = out[:, 0].min()
+        return ((out[:, 0] - t0) / 100.0, (out[:, 1] - t0) / 100.0, out[:, 2].astype(np.int64))
 
     def run(self, stream=None):
         return _lib.check(_lib.lib().gemm_amd_grouped_run(self.handle, stream.handle if stream else None),

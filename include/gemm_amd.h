@@ -278,6 +278,17 @@ int gemm_amd_grouped_run(void *plan, void *stream);
 int gemm_amd_grouped_capture(void *plan, void *stream);
 int gemm_amd_grouped_replay(void *plan, void *stream);
 int64_t gemm_amd_grouped_tiles(void *plan);
+/* Before capture (no reference counterpart; the reference's closest is the
+ * CAS tile queue of OpenBLAS-like_implementation/dgemm.c:464-482): dynamic = 1
+ * (the default of every plan) makes the workgroups claim tiles from an atomic
+ * counter (reset by a small kernel ahead of each launch, graph-capturable),
+ * dynamic = 0 gives the static stride (workgroup w takes tiles w, w + grid,
+ * ...); both compute the same bits; timeline (device memory, 3 x gemm_amd_grouped_grid(plan) uint64, or
+ * NULL) receives per workgroup its start and end on the 100 MHz real-time
+ * counter and its tile count, on every later launch.  One stream at a time
+ * per plan. */
+int gemm_amd_grouped_set_schedule(void *plan, int dynamic, unsigned long long *timeline);
+int gemm_amd_grouped_grid(void *plan);   /* workgroups of the persistent launch */
 int gemm_amd_grouped_destroy(void *plan);
 
 /* A batch of independent complex-double 3M products (zgemm3m_ contract per

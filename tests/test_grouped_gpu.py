@@ -201,6 +201,54 @@ def test_graph_replay_bitwise_stable(dtype):
     g.destroy()
 
 
+@pytest.mark.parametrize("dtype", [GROUPED_F32, GROUPED_BF16, GROUPED_F64], ids=["f32", "bf16", "f64"])
+def test_dynamic_schedule_bitwise_equal_and_replays(dtype):
+    # tiles claimed from an atomic counter are the same tiles computed the same
+    # way: the same bits as the static stride; the counter's reset kernel sits
+    # in the captured graph, so every replay starts again from tile 0
+    P = upload(random_problems(dtype, 30, seed=71 + dtype, lo=16, hi=600))
+    for p in P:
+        p["beta"] = 0.5
+    g = GroupedGemm(dtype, P, dynamic=False)
+    g.run()
+    synchronize()
+    want = [p["c"].to_host_padded() for p in P]
+    g.destroy()
+    gd = GroupedGemm(dtype, P, dynamic=True)
+    for p in P:
+        p["c"].buf.upload(p["hc"])
+    gd.run()
+    synchronize()
+    for p, w in zip(P, want):
+        assert np.array_equal(p["c"].to_host_padded(), w), ("dynamic", p["m"], p["n"], p["k"])
+    s = Stream()
+    gd.capture(s)
+    for rep in range(3):
+        for p in P:
+            p["c"].buf.upload(p["hc"])
+        synchronize()
+        gd.replay(s)
+        s.synchronize()
+        for p, w in zip(P, want):
+            assert np.array_equal(p["c"].to_host_padded(), w), ("replay", rep, p["m"], p["n"], p["k"])
+    gd.destroy()
+
+
+@pytest.mark.parametrize("dynamic", [False, True], ids=["static", "dynamic"])
+def test_timeline_accounts_for_every_tile(dynamic):
+    P = upload(random_problems(GROUPED_F32, 40, seed=81, lo=16, hi=500))
+    g = GroupedGemm(GROUPED_F32, P, dynamic=dynamic)
+    start, end, tiles = g.timeline()
+    assert len(start) == g.grid and g.grid >= 1
+    assert int(tiles.sum()) == g.tiles
+    assert np.all(end >= start) and start.min() == 0 and end.max() < 1e6   # microseconds
+    if not dynamic:   # workgroup w takes tiles w, w + grid, ...
+        assert list(tiles) == [len(range(w, g.tiles, g.grid)) for w in range(g.grid)]
+    for p in P:   # the timed launch computed the products
+        oracle_check(GROUPED_F32, p, p["c"].to_host_padded())
+    g.destroy()
+
+
 def test_replay_without_capture_raises():
     P = upload(random_problems(GROUPED_F32, 2, seed=41, lo=8, hi=32))
     g = GroupedGemm(GROUPED_F32, P)
