@@ -5,11 +5,12 @@
 //
 // The host sorts the problems by work (largest first), lays their output
 // tiles out back to back (tile_base = exclusive prefix sum) and uploads one
-// descriptor per problem.  Each workgroup of the persistent grid walks the
-// global tile sequence with a fixed stride (tile = blockIdx.x + i*gridDim.x),
-// so big tiles are spread over all CUs first and no atomic counter has to be
-// reset between launches (the launch is replayable from a captured hipGraph).
-// A tile finds its problem by binary search over tile_base and runs the same
+// descriptor per problem and a tile -> problem table.  Each workgroup of the
+// persistent grid claims tiles in that order from an atomic counter (or, with
+// the static schedule, walks them with a fixed stride), so big tiles are
+// spread over all CUs first; the launch is replayable from a captured hipGraph
+// (a one-thread kernel zeroes the counter ahead of it).
+// A tile finds its problem through the table and runs the same
 // LDS-staged matrix-core tile as the single-problem kernel, with range checks
 // at the problem's edges (sizes are arbitrary) and 16-byte vector loads for
 // problems whose operands are aligned for them.
@@ -196,36 +197,30 @@ __device__ __forceinline__ void grouped_dispatch(const GroupDesc &d, int tm, int
 // more waves resident.
 //
 // Schedules: static (queue == null), tile blockIdx.x + i * gridDim.x; or
-// dynamic (the default), each workgroup claiming its next tile from the counter *queue
+// dynamic (the default with more than two tiles per workgroup), each workgroup
+// starting on tile blockIdx.x and claiming each later one from the counter *queue
 // (zeroed by grouped_queue_reset_kernel before the launch) with one atomic
 // issued while the current tile runs, so tiles go to whichever workgroup is
 // free first (largest problems first, the list's order).  timeline (null:
 // off): per workgroup, the 100 MHz real-time counter when it starts and when
 // it has finished its last tile, and its tile count.
 template <typename TI, typename TO, typename S, typename L>
-__global__ __launch_bounds__(256) void grouped_gemm_kernel(const GroupDesc *__restrict__ descs, int count,
-                                                           int64_t total_tiles, int *__restrict__ queue,
+__global__ __launch_bounds__(256) void grouped_gemm_kernel(const GroupDesc *__restrict__ descs,
+                                                           const int *__restrict__ tilemap, int64_t total_tiles,
+                                                           int *__restrict__ queue,
                                                            unsigned long long *__restrict__ timeline) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int s_next;
+    __shared__ int64_t s_next;
     const unsigned long long t_start = timeline ? __builtin_amdgcn_s_memrealtime() : 0;
+    // the first tile of every workgroup is its own index; later tiles are
+    // claimed from the counter (offset by the grid), one claim issued behind
+    // the running tile
     int64_t t = blockIdx.x;
     int done = 0;
-    if (queue) {
-        if (threadIdx.x == 0) s_next = atomicAdd(queue, 1);
-        __syncthreads();
-        t = s_next;
-    }
     while (t < total_tiles) {
-        int claimed = 0;
-        if (queue && threadIdx.x == 0) claimed = atomicAdd(queue, 1);   // the next tile, claimed behind this one
-        int lo = 0, hi = count - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (descs[mid].tile_base <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        const GroupDesc &d = descs[lo];   // read in place (uniform scalar loads), never copied to private memory
+        int64_t claimed = 0;
+        if (queue && threadIdx.x == 0) claimed = gridDim.x + (int64_t)atomicAdd(queue, 1);
+        const GroupDesc &d = descs[tilemap[t]];   // read in place (uniform scalar loads), never copied to private memory
         const int local = (int)(t - d.tile_base);
         const int tm = local % d.tiles_m, tn = local / d.tiles_m;
         // the thread index passes through an empty asm each tile, so the tile
@@ -244,6 +239,7 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GroupDesc *__re
             if (threadIdx.x == 0) s_next = claimed;
             __syncthreads();   // the LDS stages are free again and s_next is visible
             t = s_next;
+            __syncthreads();   // every thread has read s_next before it is written again
         } else {
             lds_barrier();   // the next tile reuses the LDS stages
             t += gridDim.x;
@@ -268,13 +264,15 @@ struct GroupedPlan {
     int dtype = 0, count = 0;
     int64_t total_tiles = 0;
     GroupDesc *d_descs = nullptr;
+    int *d_tilemap = nullptr;   // problem (descriptor index) of every tile
     int grid = 0;
     size_t lds = 0;
     const void *kernel = nullptr;
     int64_t resident = 0;   // workgroups resident at once
     int *d_queue = nullptr;   // tile counter of the dynamic schedule
-    bool dynamic = true;      // the default: on the 512-problem workload the static stride left
-                              // 15-17% of the workgroup time idle in the tail
+    bool dynamic = true;      // default: on when there are more than two tiles per workgroup
+                              // (on the 512-problem workload the static stride left 15-17% of
+                              // the workgroup time idle in the tail)
     unsigned long long *timeline = nullptr;   // caller's device buffer, 3 words per workgroup
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -394,7 +392,16 @@ void *grouped_plan_create(int dtype, int count, const char *ta, const char *tb, 
         base += (int64_t)d.tiles_m * d.tiles_n;
     }
     P->total_tiles = base;
+    // tile -> problem, one word per tile: a tile finds its descriptor with one
+    // load instead of a binary search of dependent loads over the table
+    std::vector<int> tilemap((size_t)(base > 0 ? base : 1), 0);
+    for (int i = 0; i < count; ++i)
+        std::fill(tilemap.begin() + h[i].tile_base, tilemap.begin() + h[i].tile_base +
+                  (int64_t)h[i].tiles_m * h[i].tiles_n, i);
     hipError_t e = hipMalloc(&P->d_descs, sizeof(GroupDesc) * (count > 0 ? count : 1));
+    if (e == hipSuccess) e = hipMalloc(&P->d_tilemap, sizeof(int) * tilemap.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(P->d_tilemap, tilemap.data(), sizeof(int) * tilemap.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&P->d_queue, 256);
     if (e == hipSuccess && count > 0)
         e = hipMemcpy(P->d_descs, h.data(), sizeof(GroupDesc) * count, hipMemcpyHostToDevice);
@@ -407,6 +414,9 @@ void *grouped_plan_create(int dtype, int count, const char *ta, const char *tb, 
     int64_t g = P->resident;
     if (g > P->total_tiles) g = P->total_tiles;
     P->grid = (int)(g > 0 ? g : 1);
+    // the dynamic schedule pays where workgroups run several tiles of unequal
+    // cost; with about one tile each its atomics only add latency
+    P->dynamic = P->total_tiles > 2 * (int64_t)P->grid;
     if (st) {
         grouped_plan_destroy(P);
         *status = st;
@@ -423,7 +433,7 @@ int grouped_plan_launch(void *plan, hipStream_t s) {
         hipLaunchKernelGGL(grouped_queue_reset_kernel, dim3(1), dim3(64), 0, s, queue);
         if (hipError_t e = hipGetLastError()) return (int)e;
     }
-    void *args[] = {&P->d_descs, &P->count, &P->total_tiles, &queue, &P->timeline};
+    void *args[] = {&P->d_descs, &P->d_tilemap, &P->total_tiles, &queue, &P->timeline};
     return (int)hipLaunchKernel(P->kernel, dim3(P->grid), dim3(256), args, P->lds, s);
 }
 
@@ -468,6 +478,7 @@ int grouped_plan_destroy(void *plan) {
     if (P->exec) e = hipGraphExecDestroy(P->exec);
     if (P->graph) (void)hipGraphDestroy(P->graph);
     if (P->d_descs) (void)hipFree(P->d_descs);
+    if (P->d_tilemap) (void)hipFree(P->d_tilemap);
     if (P->d_queue) (void)hipFree(P->d_queue);
     delete P;
     return (int)e;

@@ -32,10 +32,12 @@ class GroupedGemm:
     dtype: GROUPED_F32, GROUPED_BF16 (operands hold bf16 bit patterns as
     uint16, C is FP32) or GROUPED_F64.  large_min: smallest M and N of a
     problem on the large tile class (None: the library's default for the
-    dtype: bf16 96, FP64 128, FP32 none).  dynamic: workgroups claim tiles
-    from an atomic counter (the default) or take the static stride."""
+    dtype: bf16 96, FP64 128, FP32 none).  dynamic: True, workgroups claim
+    tiles from an atomic counter; False, they take the static stride; None,
+    the library's choice (the counter with more than two tiles per
+    workgroup)."""
 
-    def __init__(self, dtype, problems, large_min=None, dynamic=True):
+    def __init__(self, dtype, problems, large_min=None, dynamic=None):
         self.dtype = dtype
         self.problems = list(problems)
         for p in self.problems:
@@ -60,8 +62,9 @@ class GroupedGemm:
             self.handle = L.gemm_amd_grouped_create_ex(dtype, cnt, *self._keep, int(large_min))
         if not self.handle:
             raise _lib.GemmAmdError("grouped_create: " + _lib.last_error())
-        self.dynamic = bool(dynamic)
-        _lib.check(L.gemm_amd_grouped_set_schedule(self.handle, int(self.dynamic), None), "grouped_set_schedule")
+        self.dynamic = dynamic
+        if dynamic is not None:
+            _lib.check(L.gemm_amd_grouped_set_schedule(self.handle, int(bool(dynamic)), None), "grouped_set_schedule")
 
     @property
     def tiles(self):
@@ -80,14 +83,14 @@ class GroupedGemm:
         L = _lib.lib()
         g = self.grid
         buf = DeviceBuffer(8 * 3 * g)
-        _lib.check(L.gemm_amd_grouped_set_schedule(self.handle, int(self.dynamic), ctypes.c_void_p(buf.ptr)),
-                   "grouped_set_schedule")
+        dyn = int(self.dynamic if self.dynamic is not None else self.tiles > 2 * g)
+        _lib.check(L.gemm_amd_grouped_set_schedule(self.handle, dyn, ctypes.c_void_p(buf.ptr)), "grouped_set_schedule")
         try:
             self.run(stream)
             synchronize()
             out = buf.download(np.zeros(3 * g, dtype=np.uint64)).reshape(g, 3)
         finally:
-            _lib.check(L.gemm_amd_grouped_set_schedule(self.handle, int(self.dynamic), None), "grouped_set_schedule")
+            _lib.check(L.gemm_amd_grouped_set_schedule(self.handle, dyn, None), "grouped_set_schedule")
         t0 = out[:, 0].min()
         return ((out[:, 0] - t0) / 100.0, (out[:, 1] - t0) / 100.0, out[:, 2].astype(np.int64))
 

@@ -280,10 +280,11 @@ int gemm_amd_grouped_replay(void *plan, void *stream);
 int64_t gemm_amd_grouped_tiles(void *plan);
 /* Before capture (no reference counterpart; the reference's closest is the
  * CAS tile queue of OpenBLAS-like_implementation/dgemm.c:464-482): dynamic = 1
- * (the default of every plan) makes the workgroups claim tiles from an atomic
- * counter (reset by a small kernel ahead of each launch, graph-capturable),
+ * makes the workgroups claim tiles from an atomic counter (reset by a small
+ * kernel ahead of each launch, graph-capturable) after their first one,
  * dynamic = 0 gives the static stride (workgroup w takes tiles w, w + grid,
- * ...); both compute the same bits; timeline (device memory, 3 x gemm_amd_grouped_grid(plan) uint64, or
+ * ...); both compute the same bits.  A new plan is dynamic when it has more
+ * than two tiles per workgroup.  timeline (device memory, 3 x gemm_amd_grouped_grid(plan) uint64, or
  * NULL) receives per workgroup its start and end on the 100 MHz real-time
  * counter and its tile count, on every later launch.  One stream at a time
  * per plan. */
