@@ -197,29 +197,60 @@ __device__ __forceinline__ void grouped_dispatch(const GroupDesc &d, int tm, int
 // more waves resident.
 //
 // Schedules: static (queue == null), tile blockIdx.x + i * gridDim.x; or
-// dynamic (the default with more than two tiles per workgroup), each workgroup
-// starting on tile blockIdx.x and claiming each later one from the counter *queue
-// (zeroed by grouped_queue_reset_kernel before the launch) with one atomic
-// issued while the current tile runs, so tiles go to whichever workgroup is
-// free first (largest problems first, the list's order).  timeline (null:
-// off): per workgroup, the 100 MHz real-time counter when it starts and when
-// it has finished its last tile, and its tile count.
+// dynamic (the default with more than two tiles per workgroup): the tiles are
+// dealt into 8 lists, one per XCD (workgroup b runs on XCD b mod 8), each
+// problem's tile grid cut into 8 near-square blocks so that an XCD's
+// workgroups share the A and B panels of their block in that XCD's L2; a
+// workgroup starts on entry b / 8 of its XCD's list and claims each later
+// entry from that list's counter (one atomic, issued while the current tile
+// runs), taking entries from the other XCDs' lists once its own is empty.
+// Tiles thus go to whichever workgroup is free first, largest problems first.
+// The counters (queue[16 q], one per list) are zeroed by
+// grouped_queue_reset_kernel before the launch.  timeline (null: off): per
+// workgroup, the 100 MHz real-time counter when it starts and when it has
+// finished its last tile, and its tile count.
 template <typename TI, typename TO, typename S, typename L>
 __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GroupDesc *__restrict__ descs,
                                                            const int *__restrict__ tilemap, int64_t total_tiles,
-                                                           int *__restrict__ queue,
+                                                           int *__restrict__ queue, const int *__restrict__ xlist,
+                                                           const int64_t *__restrict__ xoff,
                                                            unsigned long long *__restrict__ timeline) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int64_t s_next;
+    __shared__ int64_t s_t;
+    __shared__ int s_q;
     const unsigned long long t_start = timeline ? __builtin_amdgcn_s_memrealtime() : 0;
-    // the first tile of every workgroup is its own index; later tiles are
-    // claimed from the counter (offset by the grid), one claim issued behind
-    // the running tile
+    const int x = blockIdx.x & 7;
+    // workgroups on XCD q
+    auto nwg = [&](int q) { return (int64_t)((int)gridDim.x - q + 7) >> 3; };
+    // thread 0: the first tile at list index i of list q, or the next one any
+    // list still holds (claimed from the counters); -1 when all are empty
+    auto resolve = [&](int q, int64_t i, bool own) -> void {
+        int64_t tt = -1;
+        int qq = q;
+        for (int s = 0; s < 8; ++s) {
+            qq = (q + s) & 7;
+            if (s > 0 || !own) i = nwg(qq) + (int64_t)atomicAdd(queue + 16 * qq, 1);
+            if (i < xoff[qq + 1] - xoff[qq]) {
+                tt = xlist[xoff[qq] + i];
+                break;
+            }
+        }
+        s_t = tt;
+        s_q = qq;
+    };
     int64_t t = blockIdx.x;
+    int q = x;
+    if (queue) {
+        if (threadIdx.x == 0) resolve(x, blockIdx.x >> 3, true);
+        __syncthreads();
+        t = s_t;
+        q = s_q;
+        __syncthreads();
+    }
     int done = 0;
-    while (t < total_tiles) {
+    while (queue ? t >= 0 : t < total_tiles) {
         int64_t claimed = 0;
-        if (queue && threadIdx.x == 0) claimed = gridDim.x + (int64_t)atomicAdd(queue, 1);
+        if (queue && threadIdx.x == 0) claimed = nwg(q) + (int64_t)atomicAdd(queue + 16 * q, 1);
         const GroupDesc &d = descs[tilemap[t]];   // read in place (uniform scalar loads), never copied to private memory
         const int local = (int)(t - d.tile_base);
         const int tm = local % d.tiles_m, tn = local / d.tiles_m;
@@ -236,10 +267,19 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GroupDesc *__re
         }
         ++done;
         if (queue) {
-            if (threadIdx.x == 0) s_next = claimed;
-            __syncthreads();   // the LDS stages are free again and s_next is visible
-            t = s_next;
-            __syncthreads();   // every thread has read s_next before it is written again
+            if (threadIdx.x == 0) {
+                // the claimed entry of this list if it exists, else another list's
+                if (claimed < xoff[q + 1] - xoff[q]) {
+                    s_t = xlist[xoff[q] + claimed];
+                    s_q = q;
+                } else {
+                    resolve(q, claimed, true);
+                }
+            }
+            __syncthreads();   // the LDS stages are free again and s_t is visible
+            t = s_t;
+            q = s_q;
+            __syncthreads();   // every thread has read s_t before it is written again
         } else {
             lds_barrier();   // the next tile reuses the LDS stages
             t += gridDim.x;
@@ -253,10 +293,10 @@ __global__ __launch_bounds__(256) void grouped_gemm_kernel(const GroupDesc *__re
     }
 }
 
-// Zeroes the dynamic schedule's tile counter (a kernel, not a memset, so it
-// can sit in a captured graph ahead of the grouped launch).
+// Zeroes the dynamic schedule's 8 tile counters (a kernel, not a memset, so
+// it can sit in a captured graph ahead of the grouped launch).
 __global__ __launch_bounds__(64) void grouped_queue_reset_kernel(int *queue) {
-    if (threadIdx.x == 0) *queue = 0;
+    if (threadIdx.x < 8) queue[16 * threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------- host side
@@ -269,7 +309,9 @@ struct GroupedPlan {
     size_t lds = 0;
     const void *kernel = nullptr;
     int64_t resident = 0;   // workgroups resident at once
-    int *d_queue = nullptr;   // tile counter of the dynamic schedule
+    int *d_queue = nullptr;   // the dynamic schedule's 8 tile counters (64 bytes apart)
+    int *d_xlist = nullptr;   // the dynamic schedule's tile lists, one per XCD, back to back
+    int64_t *d_xoff = nullptr;   // list q = d_xlist[d_xoff[q] .. d_xoff[q+1])
     bool dynamic = true;      // default: on when there are more than two tiles per workgroup
                               // (on the 512-problem workload the static stride left 15-17% of
                               // the workgroup time idle in the tail)
@@ -402,7 +444,62 @@ void *grouped_plan_create(int dtype, int count, const char *ta, const char *tb, 
     if (e == hipSuccess) e = hipMalloc(&P->d_tilemap, sizeof(int) * tilemap.size());
     if (e == hipSuccess)
         e = hipMemcpy(P->d_tilemap, tilemap.data(), sizeof(int) * tilemap.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&P->d_queue, 256);
+    if (e == hipSuccess) e = hipMalloc(&P->d_queue, 8 * 16 * sizeof(int));
+    // the dynamic schedule's per-XCD lists: each problem's tile grid in 8
+    // near-square blocks (problems of fewer than 16 tiles dealt tile by tile),
+    // the blocks' XCDs rotated from problem to problem so the lists even out
+    if (e == hipSuccess) {
+        std::vector<std::vector<int>> lists(8);
+        int rot = 0;
+        for (int i = 0; i < count; ++i) {
+            const GroupDesc &d = h[i];
+            const int tm = d.tiles_m, tn = d.tiles_n, nt = tm * tn;
+            if (nt < 16) {
+                for (int j = 0; j < nt; ++j) lists[(rot + j) & 7].push_back((int)(d.tile_base + j));
+                rot = (rot + nt) & 7;
+                continue;
+            }
+            int pm = 1, pn = 8;
+            double bestr = 1e30;
+            for (int a = 1; a <= 8; a *= 2) {   // blocks of (tm / a) x (tn / (8 / a)) tiles, nearest to square
+                const int b = 8 / a;
+                if (a > tm || b > tn) continue;
+                const double r = (double)tm / a / ((double)tn / b);
+                const double dev = r > 1 ? r : 1 / r;
+                if (dev < bestr) {
+                    bestr = dev;
+                    pm = a;
+                    pn = b;
+                }
+            }
+            if (bestr == 1e30) {   // thin grids (fewer than 8 tiles in both directions' splits): deal tiles
+                for (int j = 0; j < nt; ++j) lists[(rot + j) & 7].push_back((int)(d.tile_base + j));
+                rot = (rot + nt) & 7;
+                continue;
+            }
+            for (int bj = 0; bj < pn; ++bj)
+                for (int bi = 0; bi < pm; ++bi) {
+                    const int q = (bi * pn + bj + rot) & 7;
+                    const int m0 = bi * tm / pm, m1 = (bi + 1) * tm / pm;
+                    const int n0 = bj * tn / pn, n1 = (bj + 1) * tn / pn;
+                    for (int c = n0; c < n1; ++c)
+                        for (int r = m0; r < m1; ++r) lists[q].push_back((int)(d.tile_base + r + (int64_t)c * tm));
+                }
+            rot = (rot + 1) & 7;
+        }
+        std::vector<int> flat;
+        std::vector<int64_t> off(9, 0);
+        for (int q = 0; q < 8; ++q) {
+            off[q] = (int64_t)flat.size();
+            flat.insert(flat.end(), lists[q].begin(), lists[q].end());
+        }
+        off[8] = (int64_t)flat.size();
+        if (flat.empty()) flat.push_back(0);
+        e = hipMalloc(&P->d_xlist, sizeof(int) * flat.size());
+        if (e == hipSuccess) e = hipMemcpy(P->d_xlist, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&P->d_xoff, sizeof(int64_t) * 9);
+        if (e == hipSuccess) e = hipMemcpy(P->d_xoff, off.data(), sizeof(int64_t) * 9, hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess && count > 0)
         e = hipMemcpy(P->d_descs, h.data(), sizeof(GroupDesc) * count, hipMemcpyHostToDevice);
     int st = (int)e;
@@ -433,7 +530,7 @@ int grouped_plan_launch(void *plan, hipStream_t s) {
         hipLaunchKernelGGL(grouped_queue_reset_kernel, dim3(1), dim3(64), 0, s, queue);
         if (hipError_t e = hipGetLastError()) return (int)e;
     }
-    void *args[] = {&P->d_descs, &P->d_tilemap, &P->total_tiles, &queue, &P->timeline};
+    void *args[] = {&P->d_descs, &P->d_tilemap, &P->total_tiles, &queue, &P->d_xlist, &P->d_xoff, &P->timeline};
     return (int)hipLaunchKernel(P->kernel, dim3(P->grid), dim3(256), args, P->lds, s);
 }
 
@@ -480,6 +577,8 @@ int grouped_plan_destroy(void *plan) {
     if (P->d_descs) (void)hipFree(P->d_descs);
     if (P->d_tilemap) (void)hipFree(P->d_tilemap);
     if (P->d_queue) (void)hipFree(P->d_queue);
+    if (P->d_xlist) (void)hipFree(P->d_xlist);
+    if (P->d_xoff) (void)hipFree(P->d_xoff);
     delete P;
     return (int)e;
 }
