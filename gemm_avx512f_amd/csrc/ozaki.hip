@@ -1043,11 +1043,21 @@ __global__ __launch_bounds__(256, 1) void ozaki_crt_wide_kernel(CrtParams p) {
 // CRT reconstruction after ozaki_crt_wide_kernel, one thread per (wave,
 // 32x32 tile, lane) of the product kernel: the thread loads that lane's
 // residues of the tile under every modulus at once (16 bytes per modulus,
-// 1 KB per modulus across a wave) and rebuilds its 16 elements of C.  Run as
+// 1 KB per modulus across a wave) and rebuilds its 16 elements of V.  Run as
 // its own launch it has the whole chip and many waves in flight for what is
 // a memory-bound pass; inside the product kernel it ran with one wave per
 // SIMD and nothing to hide its loads behind.
+//
+// A workgroup rebuilds a 32 x 128 strip of C (wave b: columns 32b..32b+31).
+// A lane's 16 elements lie in one column, so writing them straight to
+// column-major C put each store instruction's 64 lanes on 32 columns (8-byte
+// pieces of 64 cache lines; the launch wrote four times C's size to HBM).
+// The strip passes through LDS instead and each wave then writes two
+// columns' 32 contiguous rows per instruction; the exponents, alpha and beta
+// are applied in that pass (one row exponent per lane).
 __global__ __launch_bounds__(256) void ozaki_crt_rebuild_kernel(CrtParams p) {
+    constexpr int SP = 33;   // strip pitch in doubles (column-major, odd: conflict-free row reads)
+    __shared__ double S[128 * SP];
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int lane = (int)(t & 63), ab = (int)((t >> 6) & 15);
     const int64_t wv = t >> 10;
@@ -1060,10 +1070,8 @@ __global__ __launch_bounds__(256) void ozaki_crt_rebuild_kernel(CrtParams p) {
     i4_t R[OZ_CRT_MAXMOD];
 #pragma unroll
     for (int l = 0; l < OZ_CRT_MAXMOD; ++l) R[l] = l < nmod ? Rv[(int64_t)l * 16 * 64] : i4_t{0, 0, 0, 0};
-    const int j = n0 + 32 * b + q;
-    if (j >= p.N) return;
-    const int eb = p.eb[j];
     const double two32 = 4294967296.0;
+    double *col = S + (32 * b + q) * SP;
 #pragma unroll
     for (int e4 = 0; e4 < 4; ++e4) {
         double s[4][4];
@@ -1088,8 +1096,6 @@ __global__ __launch_bounds__(256) void ozaki_crt_rebuild_kernel(CrtParams p) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int ee = 4 * e4 + e;
-            const int i = m0 + 32 * a + (ee >> 2) * 8 + g * 4 + (ee & 3);
-            if (i >= p.M) continue;
             const double approx = (s[e][3] * two32 + s[e][2]) * (two32 * two32) + s[e][1] * two32;
             const double qq = rint(approx * p.cc->invM);
             long long t0 = (long long)fma(-qq, p.cc->Mk[0], s[e][0]);
@@ -1112,10 +1118,23 @@ __global__ __launch_bounds__(256) void ozaki_crt_rebuild_kernel(CrtParams p) {
             const unsigned long long ulo = ((unsigned long long)t1 << 32) | (unsigned long long)t0;
             const long long lo = (long long)ulo;
             if (ulo >> 63) hi += 1;
-            const double V = fma((double)hi, two32 * two32, (double)lo);
-            const double v = p.alpha * ldexp(V, p.ea[i] + eb);
-            double *dst = p.C + i + (int64_t)j * p.ldc;
-            *dst = (p.beta == 0.0) ? v : v + p.beta * *dst;
+            col[(ee >> 2) * 8 + g * 4 + (ee & 3)] = fma((double)hi, two32 * two32, (double)lo);
+        }
+    }
+    __syncthreads();
+    // write pass: wave w, round u: columns 8u + 2w + g, row q
+    const int i = m0 + 32 * a + q;
+    const int wq = threadIdx.x >> 6;
+    if (i < p.M) {
+        const int ea = p.ea[i];
+#pragma unroll 4
+        for (int u = 0; u < 16; ++u) {
+            const int cl = 8 * u + 2 * wq + g, j = n0 + cl;
+            if (j < p.N) {
+                const double v = p.alpha * ldexp(S[cl * SP + q], ea + p.eb[j]);
+                double *dst = p.C + i + (int64_t)j * p.ldc;
+                *dst = (p.beta == 0.0) ? v : v + p.beta * *dst;
+            }
         }
     }
 }
