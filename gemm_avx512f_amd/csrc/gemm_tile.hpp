@@ -371,6 +371,28 @@ struct WaveF32 {
                 for (int e = 0; e < 16; ++e)
                     visit(wm0 + sm * 32 + i, wn0 + sn * 32 + (e >> 2) * 8 + g * 4 + (e & 3), acc[sm][sn][e]);
     }
+
+    // C = alpha * acc for a tile wholly inside C (no range checks), c at the
+    // tile's (0, 0): a lane's elements lie on its row at column offsets that
+    // are constants times ldc, so each store is one address add
+    template <typename P>
+    __device__ __forceinline__ void store_full(P c, int64_t ldc, int lane, int wm0, int wn0, float alpha) const {
+        store_rows(acc, c, ldc, lane, wm0, wn0, alpha);
+    }
+    template <typename A, typename P>
+    __device__ __forceinline__ static void store_rows(const A &acc, P c, int64_t ldc, int lane, int wm0, int wn0,
+                                                      float alpha) {
+        const int i = lane & 31, g = lane >> 5;
+#pragma unroll
+        for (int sm = 0; sm < SM; ++sm) {
+            const P row = c + (wm0 + sm * 32 + i) + (int64_t)(wn0 + g * 4) * ldc;
+#pragma unroll
+            for (int sn = 0; sn < SN; ++sn)
+#pragma unroll
+                for (int e = 0; e < 16; ++e)
+                    row[(int64_t)(sn * 32 + (e >> 2) * 8 + (e & 3)) * ldc] = alpha * acc[sm][sn][e];
+        }
+    }
 };
 
 // BF16 inputs, FP32 accumulation: v_mfma_f32_32x32x16_bf16.  Operands are bf16
@@ -473,6 +495,12 @@ struct WaveBF16 {
 #pragma unroll
                 for (int e = 0; e < 16; ++e)
                     visit(wm0 + sm * 32 + i, wn0 + sn * 32 + (e >> 2) * 8 + g * 4 + (e & 3), acc[sm][sn][e]);
+    }
+
+    // as WaveF32::store_full (the same accumulator map)
+    template <typename P>
+    __device__ __forceinline__ void store_full(P c, int64_t ldc, int lane, int wm0, int wn0, float alpha) const {
+        WaveF32<WTM, WTN, BK>::store_rows(acc, c, ldc, lane, wm0, wn0, alpha);
     }
 };
 

@@ -167,12 +167,26 @@ __device__ __forceinline__ void grouped_tile(const GroupDesc &d, int tm, int tn,
     }
     auto cbase = as_global(static_cast<TO *>(d.C)) + m0 + (int64_t)n0 * d.ldc;
     const TO alpha = (TO)d.alpha, beta = (TO)d.beta;
-    w.for_each(lane, wm0, wn0, [&](int r, int c, TO v) {
-        if (r < mlim && c < nlim) {
-            auto dst = cbase + r + (int64_t)c * d.ldc;
-            *dst = axpby(alpha, v, beta, dst);
+    // beta is tested once per tile (its descriptor value is uniform); a tile
+    // wholly inside C with beta == 0 stores without range checks
+    if constexpr (!DP) {
+        if (d.beta == 0.0 && mlim >= BM && nlim >= BN) {
+            w.store_full(cbase, d.ldc, lane, wm0, wn0, alpha);
+            return;
         }
-    });
+    }
+    if (d.beta == 0.0) {
+        w.for_each(lane, wm0, wn0, [&](int r, int c, TO v) {
+            if (r < mlim && c < nlim) cbase[r + (int64_t)c * d.ldc] = alpha * v;
+        });
+    } else {
+        w.for_each(lane, wm0, wn0, [&](int r, int c, TO v) {
+            if (r < mlim && c < nlim) {
+                auto dst = cbase + r + (int64_t)c * d.ldc;
+                *dst = axpby(alpha, v, beta, dst);
+            }
+        });
+    }
 }
 
 // One tile of shape BM x BN (slab depth BK) in the variant the problem needs.
