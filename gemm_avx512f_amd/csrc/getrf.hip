@@ -30,6 +30,62 @@ __device__ inline bool pivot_better(double m1, int r1, double m2, int r2) {
     return m1 > m2 || (m1 == m2 && r1 < r2);
 }
 
+// One step of a pivot reduction within 16-lane rows: the candidate of the
+// lane that the DPP pattern CTRL names (a data-parallel lane move, no LDS
+// round trip) against the lane's own.
+template <int CTRL>
+__device__ __forceinline__ void pivot_dpp(double &best, int &brow) {
+    const long long bits = __double_as_longlong(best);
+    const int lo = (int)bits, hi = (int)(bits >> 32);
+    const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+    const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+    const int orow = __builtin_amdgcn_update_dpp(brow, brow, CTRL, 0xF, 0xF, false);
+    const double om = __longlong_as_double((long long)(((unsigned long long)(unsigned)ohi << 32) | (unsigned)olo));
+    if (pivot_better(om, orow, best, brow)) {
+        best = om;
+        brow = orow;
+    }
+}
+
+// Every lane of a 16-lane row ends with the row's best candidate: exchanges
+// with lane ^ 1 and lane ^ 2 (quad permutes), then with the mirrored lane in
+// the half row and in the row, each pairing two halves that already hold
+// their own best.
+__device__ __forceinline__ void pivot_row_reduce(double &best, int &brow) {
+    pivot_dpp<0xB1>(best, brow);    // quad_perm [1, 0, 3, 2]
+    pivot_dpp<0x4E>(best, brow);    // quad_perm [2, 3, 0, 1]
+    pivot_dpp<0x141>(best, brow);   // row_half_mirror
+    pivot_dpp<0x140>(best, brow);   // row_mirror
+}
+
+// The wave's best candidate (every lane gets it): the four rows' results,
+// read as wave-uniform values.  The order on (|a|, row) is total (NaN
+// magnitudes never become candidates), so the result does not depend on the
+// reduction's shape.
+__device__ __forceinline__ void pivot_wave_reduce(double &best, int &brow) {
+    pivot_row_reduce(best, brow);
+    const long long bits = __double_as_longlong(best);
+    const int lo = (int)bits, hi = (int)(bits >> 32);
+    double b = best;
+    int r = brow;
+    b = __longlong_as_double((long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, 0) << 32) |
+                                         (unsigned)__builtin_amdgcn_readlane(lo, 0)));
+    r = __builtin_amdgcn_readlane(brow, 0);
+#pragma unroll
+    for (int row = 1; row < 4; ++row) {
+        const double ob = __longlong_as_double(
+            (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, 16 * row) << 32) |
+                        (unsigned)__builtin_amdgcn_readlane(lo, 16 * row)));
+        const int orow = __builtin_amdgcn_readlane(brow, 16 * row);
+        if (pivot_better(ob, orow, b, r)) {
+            b = ob;
+            r = orow;
+        }
+    }
+    best = b;
+    brow = r;
+}
+
 // Factor columns [c0, c0 + w) of A over rows [c0, m) (T rows per thread, at
 // most W columns); getrf_swap_kernel then applies the w interchanges to the
 // other columns.  ipiv[c] receives the 1-based pivot row of column c; info the
@@ -84,33 +140,19 @@ __global__ __launch_bounds__(LU_THREADS) void getrf_leaf_kernel(double *__restri
                 brow = r;
             }
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const double om = __shfl_xor(best, off);
-            const int orow = __shfl_xor(brow, off);
-            if (pivot_better(om, orow, best, brow)) {
-                best = om;
-                brow = orow;
-            }
-        }
+        pivot_wave_reduce(best, brow);
         if (lane == 0) {
             s_mag[b][wave] = best;
             s_row[b][wave] = brow;
         }
         __syncthreads();
-        // 2. every wave reduces the 16 partials; the pivot row's owner publishes it
+        // 2. every wave reduces the 16 partials (lanes 0-15, one 16-lane row);
+        // the pivot row's owner publishes it
+        static_assert(LU_WAVES == 16, "the partials fill one 16-lane row");
         double bm = lane < LU_WAVES ? s_mag[b][lane] : -1.0;
         int br = lane < LU_WAVES ? s_row[b][lane] : 0x7fffffff;
-#pragma unroll
-        for (int off = LU_WAVES / 2; off > 0; off >>= 1) {
-            const double om = __shfl_xor(bm, off);
-            const int orow = __shfl_xor(br, off);
-            if (pivot_better(om, orow, bm, br)) {
-                bm = om;
-                br = orow;
-            }
-        }
-        br = __shfl(br, 0);
+        pivot_row_reduce(bm, br);
+        br = __builtin_amdgcn_readlane(br, 0);
         if (br >= m) br = cj;   // no candidate compared larger (NaN column): keep the diagonal row
         const int p = br;
         if (tid == 0) s_ipiv[j] = p + 1;

@@ -229,12 +229,19 @@ int gemm_amd_memcpy(void *dst, const void *src, size_t bytes, int kind) {
     if (int st = init_status()) return st;
     hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost
                                                                      : hipMemcpyDeviceToDevice;
-    return hip_status(hipMemcpy(dst, src, bytes, k), "hipMemcpy");
+    hipError_t e = hipMemcpy(dst, src, bytes, k);
+    // a device-to-device hipMemcpy may return before the copy has run; the
+    // entry point is synchronous, so wait for it (kernels on non-blocking
+    // streams do not wait for the null stream)
+    if (e == hipSuccess && k == hipMemcpyDeviceToDevice) e = hipStreamSynchronize(nullptr);
+    return hip_status(e, "hipMemcpy");
 }
 
 int gemm_amd_memset(void *p, int value, size_t bytes) {
     if (int st = init_status()) return st;
-    return hip_status(hipMemset(p, value, bytes), "hipMemset");
+    hipError_t e = hipMemset(p, value, bytes);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);   // synchronous, like gemm_amd_memcpy
+    return hip_status(e, "hipMemset");
 }
 
 int gemm_amd_synchronize(void) {

@@ -355,7 +355,7 @@ void *gemm_amd_malloc(size_t bytes);              /* device memory, NULL on fail
 int gemm_amd_free(void *dptr);
 /* kind: 1 host->device, 2 device->host, 3 device->device; synchronous */
 int gemm_amd_memcpy(void *dst, const void *src, size_t bytes, int kind);
-int gemm_amd_memset(void *dptr, int value, size_t bytes);
+int gemm_amd_memset(void *dptr, int value, size_t bytes);   /* synchronous */
 int gemm_amd_synchronize(void);
 
 void *gemm_amd_stream_create(void);               /* default-priority stream */

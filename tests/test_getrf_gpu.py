@@ -154,3 +154,31 @@ def test_argument_errors(gpu_lib, capfd):
     info = np.zeros(1, np.int32)
     dinfo.download(info)
     assert info[0] == 0
+
+
+def test_refactor_after_device_copy_is_repeatable(gpu_lib):
+    """8192^2, as bench.py times it: the matrix is restored with a device-to-device
+    gemm_amd_memcpy before each factorisation, which runs on a non-blocking
+    stream.  The copy must be complete when the entry point returns (the
+    factorisation's first row interchange spans every column): the second
+    factorisation gives the first one's bits."""
+    import ctypes
+
+    from gemm_avx512f_amd.device import Stream, fill_uniform
+
+    n = 8192
+    stream = Stream()
+    A0, A = DeviceMatrix(n, n, np.float64), DeviceMatrix(n, n, np.float64)
+    fill_uniform(A0, 21, 0, stream=stream)
+    stream.synchronize()
+    piv, info = DeviceBuffer(4 * n), DeviceBuffer(4)
+    outs = []
+    for _ in range(2):
+        _lib.check(_lib.lib().gemm_amd_memcpy(ctypes.c_void_p(A.ptr), ctypes.c_void_p(A0.ptr), 8 * n * n, 3), "copy")
+        blas.dgetrf(n, n, A, piv, info, stream=stream)
+        stream.synchronize()
+        ipiv = np.zeros(n, np.int32)
+        piv.download(ipiv)
+        outs.append((A.to_host(), ipiv))
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(outs[0][0], outs[1][0])
