@@ -246,32 +246,17 @@ __global__ __launch_bounds__(1024) void getrf_leaf_global_kernel(double *__restr
                 brow = r;
             }
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const double om = __shfl_xor(best, off);
-            const int orow = __shfl_xor(brow, off);
-            if (pivot_better(om, orow, best, brow)) {
-                best = om;
-                brow = orow;
-            }
-        }
+        pivot_wave_reduce(best, brow);
         if (lane == 0) {
             s_mag[b][wave] = best;
             s_row[b][wave] = brow;
         }
         __syncthreads();
+        static_assert(NW == 16, "the partials fill one 16-lane row");
         double bm = lane < NW ? s_mag[b][lane] : -1.0;
         int br = lane < NW ? s_row[b][lane] : 0x7fffffff;
-#pragma unroll
-        for (int off = NW / 2; off > 0; off >>= 1) {
-            const double om = __shfl_xor(bm, off);
-            const int orow = __shfl_xor(br, off);
-            if (pivot_better(om, orow, bm, br)) {
-                bm = om;
-                br = orow;
-            }
-        }
-        br = __shfl(br, 0);
+        pivot_row_reduce(bm, br);
+        br = __builtin_amdgcn_readlane(br, 0);
         const int p = br >= m ? cj : br;   // no candidate compared larger (NaN column): keep the diagonal row
         const int powner = (p - c0) % NT;
         if (tid == 0) s_ipiv[j] = p + 1;

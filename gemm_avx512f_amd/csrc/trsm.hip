@@ -35,16 +35,18 @@ __device__ __forceinline__ double opa(const double *A, int64_t lda, bool trans, 
     return trans ? A[c + r * lda] : A[r + c * lda];
 }
 
-// grid.x = diagonal block index; one wave per block, lane c computes column c
-// of the inverse by substitution against e_c with the column in registers and
-// the block's rows read from LDS (every lane reads the same entry: a
-// broadcast).  W[b] (NB x NB, ld NB) <- inv(op(A)_bb); rows/cols past the
-// matrix edge are the identity.  Block 0 also writes the NB zeros that follow
-// the last block.
-__global__ __launch_bounds__(64) void trsm_diag_inverse_kernel(const double *__restrict__ A, int64_t lda, int na,
-                                                               bool trans, bool oplower, bool unit,
-                                                               double *__restrict__ W) {
-    constexpr int NB = TRSM_NB;
+// grid.x = diagonal block index, 256 threads per block.  All four waves load
+// the block into LDS at once (16 loads in flight per thread, one memory
+// latency); wave 0 then inverts it: lane c computes column c of the inverse
+// by substitution against e_c with the column in registers and the block's
+// rows read from LDS (every lane reads the same entry: a broadcast), in four
+// independent FMA chains per row; all four waves write it out.  W[b] (NB x
+// NB, ld NB) <- inv(op(A)_bb); rows/cols past the matrix edge are the
+// identity.  Block 0 also writes the NB zeros that follow the last block.
+__global__ __launch_bounds__(256) void trsm_diag_inverse_kernel(const double *__restrict__ A, int64_t lda, int na,
+                                                                bool trans, bool oplower, bool unit,
+                                                                double *__restrict__ W) {
+    constexpr int NB = TRSM_NB, NT = 256;
     // op(A) block [row][col], later the inverse; the pitch keeps rows 16-byte
     // aligned for paired broadcast reads and spreads column accesses over banks
     __shared__ __align__(16) double T[NB][NB + 2];
@@ -52,61 +54,65 @@ __global__ __launch_bounds__(64) void trsm_diag_inverse_kernel(const double *__r
     const int b = blockIdx.x;
     const int64_t base = (int64_t)b * NB;
     const int nb = min(NB, na - (int)base);
-    const int c = threadIdx.x;
-    for (int c0 = 0; c0 < NB; c0 += 16) {   // lane = the contiguous index; 16 loads in flight
-        double v[16];
+    const int tid = threadIdx.x, lane = tid & 63;
+    {
+        // element e = tid + NT u: its contiguous index (row for op(A) = A, column
+        // for op(A) = A^T) is the lane, so every load is coalesced
+        double v[NB * NB / NT];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int r = trans ? c0 + u : c, col = trans ? c : c0 + u;
+        for (int u = 0; u < NB * NB / NT; ++u) {
+            const int o = (tid + NT * u) >> 6;
+            const int r = trans ? o : lane, col = trans ? lane : o;
             const bool in_tri = oplower ? (r > col) : (r < col);
             v[u] = (r < nb && col < nb && in_tri) ? opa(A, lda, trans, base + r, base + col) : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int r = trans ? c0 + u : c, col = trans ? c : c0 + u;
+        for (int u = 0; u < NB * NB / NT; ++u) {
+            const int o = (tid + NT * u) >> 6;
+            const int r = trans ? o : lane, col = trans ? lane : o;
             T[r][col] = v[u];
         }
     }
-    {
+    if (tid < NB) {
         double d = 1.0;
-        if (c < nb && !unit) d = opa(A, lda, trans, base + c, base + c);
-        piv[c] = 1.0 / d;
+        if (tid < nb && !unit) d = opa(A, lda, trans, base + tid, base + tid);
+        piv[tid] = 1.0 / d;
     }
     __syncthreads();
-    double x[NB];
-    if (oplower) {
+    if (tid < 64) {   // wave 0 (no barrier inside)
+        const int c = lane;
+        double x[NB];
+        if (oplower) {
 #pragma unroll
-        for (int i = 0; i < NB; ++i) {
-            double s0 = (i == c) ? 1.0 : 0.0, s1 = 0.0;
+            for (int i = 0; i < NB; ++i) {
+                double s[4] = {(i == c) ? 1.0 : 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int j = 0; j + 1 < i; j += 2) {
-                s0 = fma(-T[i][j], x[j], s0);
-                s1 = fma(-T[i][j + 1], x[j + 1], s1);
+                for (int j = 0; j < i; ++j) s[j & 3] = fma(-T[i][j], x[j], s[j & 3]);
+                x[i] = ((s[0] + s[1]) + (s[2] + s[3])) * piv[i];
             }
-            if (i & 1) s0 = fma(-T[i][i - 1], x[i - 1], s0);
-            x[i] = (s0 + s1) * piv[i];
-        }
-    } else {
+        } else {
 #pragma unroll
-        for (int i = NB - 1; i >= 0; --i) {
-            double s0 = (i == c) ? 1.0 : 0.0, s1 = 0.0;
-            const int n = NB - 1 - i;   // terms j = i+1 .. NB-1
+            for (int i = NB - 1; i >= 0; --i) {
+                double s[4] = {(i == c) ? 1.0 : 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int t = 0; t + 1 < n; t += 2) {
-                s0 = fma(-T[i][i + 1 + t], x[i + 1 + t], s0);
-                s1 = fma(-T[i][i + 2 + t], x[i + 2 + t], s1);
+                for (int j = i + 1; j < NB; ++j) s[(j - i - 1) & 3] = fma(-T[i][j], x[j], s[(j - i - 1) & 3]);
+                x[i] = ((s[0] + s[1]) + (s[2] + s[3])) * piv[i];
             }
-            if (n & 1) s0 = fma(-T[i][NB - 1], x[NB - 1], s0);
-            x[i] = (s0 + s1) * piv[i];
         }
+        // T is read by every lane of this wave only: its writes follow its reads
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int r = 0; r < NB; ++r) T[r][c] = x[r];
     }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < NB; ++r) T[r][c] = x[r];
     __syncthreads();
     double *out = W + (int64_t)b * NB * NB;
-    for (int cc = 0; cc < NB; ++cc) out[c + (int64_t)cc * NB] = T[c][cc];
-    if (b == 0) W[(int64_t)gridDim.x * NB * NB + c] = 0.0;   // the zero operand of the panel kernel
+#pragma unroll
+    for (int u = 0; u < NB * NB / NT; ++u) {
+        const int e = tid + NT * u;
+        out[e] = T[e & 63][e >> 6];
+    }
+    if (b == 0 && tid < NB) W[(int64_t)gridDim.x * NB * NB + tid] = 0.0;   // the zero operand of the panel kernel
 }
 
 // B (rows x cols, ldb) <- alpha * B; alpha == 0 writes zeros without reading.
@@ -383,7 +389,7 @@ int launch_dtrsm(bool left, bool upper, bool trans, bool unit, int m, int n, dou
     // op(A)^T (the other transpose, the other triangle) for side R.
     const bool inv_trans = left ? trans : !trans, inv_lower = left ? oplower : !oplower;
     const int nblk = (na + TRSM_NB - 1) / TRSM_NB;
-    hipLaunchKernelGGL(trsm_diag_inverse_kernel, dim3(nblk), dim3(64), 0, s, A, lda, na, inv_trans, inv_lower, unit,
+    hipLaunchKernelGGL(trsm_diag_inverse_kernel, dim3(nblk), dim3(256), 0, s, A, lda, na, inv_trans, inv_lower, unit,
                        inv_ws);
     if (hipError_t e = hipGetLastError()) return (int)e;
     if (alpha != 1.0) {
