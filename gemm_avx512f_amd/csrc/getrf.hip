@@ -86,121 +86,147 @@ __device__ __forceinline__ void pivot_wave_reduce(double &best, int &brow) {
     brow = r;
 }
 
-// Factor columns [c0, c0 + w) of A over rows [c0, m) (T rows per thread, at
-// most W columns); getrf_swap_kernel then applies the w interchanges to the
-// other columns.  ipiv[c] receives the 1-based pivot row of column c; info the
-// first (1-based) column whose pivot is exactly zero, as dgetf2 reports it.
-template <int T, int W>
+// Factor columns [c0, c0 + w) of A over rows [c0, m) (T rows per thread,
+// row c0 + i in register row i / NT of thread i % NT, where NT = blockDim.x is
+// a multiple of 64 between 64 and 1024 sized to the panel; at most W columns;
+// FULL: w == W); getrf_swap_kernel then applies the w interchanges to the
+// other columns.  ipiv[c] receives the 1-based pivot row
+// of column c; info the first (1-based) column whose pivot is exactly zero,
+// as dgetf2 reports it.
+//
+// One barrier per column.  Before it, every wave publishes its best pivot
+// candidate together with that row's panel entries (the lane that owns the
+// row writes them), and thread j publishes row c0+j.  After it, every wave
+// reduces the candidates and takes the pivot row from the winning wave's
+// slot, so no second exchange is needed.  Slots are double buffered by
+// column parity: column j+2 rewrites column j's slots only after every wave
+// has passed column j+1's barrier, that is, after it finished reading them.
+// The column step is bound by instruction issue (up to 16 waves share the
+// CU's four SIMDs and each repeats the reductions), so the workgroup has no
+// more waves than the panel's rows need, conditions that hold for every row
+// but the first few (t > 0 rows lie below any diagonal of the leaf) are
+// spelled out, and a full-width leaf tests no column bounds.
+template <int T, int W, bool FULL>
 __global__ __launch_bounds__(LU_THREADS) void getrf_leaf_kernel(double *__restrict__ A, int64_t lda, int m, int c0,
                                                                 int w, int *__restrict__ ipiv,
                                                                 int *__restrict__ info) {
-    // Two barriers per column: partials, pivot row and row cj are double
-    // buffered by column parity, so a column's reads never meet the next
-    // column's writes.
+    static_assert(W <= 64, "rows c0 .. c0+W-1 are the first register rows of threads 0 .. W-1 (NT >= 64)");
     __shared__ double s_mag[2][LU_WAVES];
     __shared__ int s_row[2][LU_WAVES];
-    __shared__ double s_prow[2][W];   // the pivot row's panel entries (become row c0+j)
-    __shared__ double s_crow[2][W];   // row c0+j's panel entries (move to the pivot row)
-    __shared__ int s_ipiv[W];          // pivots of the leaf, stored to ipiv after the loop (no global
-                                       // store may be outstanding at a column's barriers)
+    __shared__ double s_cand[2][LU_WAVES][W];   // each wave's best candidate row (panel entries)
+    __shared__ double s_crow[2][W];             // row c0+j's panel entries (move to the pivot row)
+    __shared__ int s_ipiv[W];                    // pivots of the leaf, stored to ipiv after the loop (no global
+                                                 // store may be outstanding at a column's barrier)
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int NT = blockDim.x, nwaves = NT >> 6;
     int zero_col = -1;   // thread 0: first column of this leaf with an exactly zero pivot
+    auto incol = [&](int k) { return FULL ? k < W : k < w; };
 
     double v[T][W];
+    bool live[T];   // the register row lies inside the matrix
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-        const int r = c0 + tid + t * LU_THREADS;
+        const int r = c0 + tid + t * NT;
+        live[t] = r < m;
 #pragma unroll
-        for (int k = 0; k < W; ++k) v[t][k] = (r < m && k < w) ? A[r + (int64_t)(c0 + k) * lda] : 0.0;
+        for (int k = 0; k < W; ++k) v[t][k] = (live[t] && incol(k)) ? A[r + (int64_t)(c0 + k) * lda] : 0.0;
     }
 
     // Unrolled over the panel width so that the column index is a constant:
     // the register panel is then addressed directly, without select chains.
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int j = 0; j < W; ++j) {
-        if (j >= w) continue;   // w is uniform: every thread skips the same columns
+        if (!FULL && j >= w) continue;   // w is uniform: every thread skips the same columns
         const int cj = c0 + j, b = j & 1;
-        // 1. row cj's entries; pivot candidates over rows >= cj (first maximum of |a|)
+        // 1. row cj (thread j's first register row); pivot candidates over the
+        // thread's rows >= cj: rows ascend with t, so a strict comparison keeps
+        // the first maximum of |a| (a NaN magnitude never wins)
+        if (tid == j)
+#pragma unroll
+            for (int k = 0; k < W; ++k) s_crow[b][k] = v[0][k];
         double best = -1.0;
         int brow = 0x7fffffff;
 #pragma unroll
         for (int t = 0; t < T; ++t) {
-            const int r = c0 + tid + t * LU_THREADS;
-            if (r == cj)
-#pragma unroll
-                for (int k = 0; k < W; ++k) s_crow[b][k] = v[t][k];
-            double a = 0.0;
-#pragma unroll
-            for (int k = 0; k < W; ++k)
-                if (k == j) a = v[t][k];
-            const double mag = fabs(a);
-            if (r >= cj && r < m && pivot_better(mag, r, best, brow)) {
+            const int r = c0 + tid + t * NT;
+            const double mag = fabs(v[t][j]);
+            if ((t > 0 || r >= cj) && live[t] && mag > best) {
                 best = mag;
                 brow = r;
             }
         }
         pivot_wave_reduce(best, brow);
+        // the winning row's owner publishes its entries (tested row by row:
+        // a test on the register row's index would let the compiler index
+        // the register panel dynamically)
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+            if (c0 + tid + t * NT == brow)
+#pragma unroll
+                for (int k = 0; k < W; ++k) s_cand[b][wave][k] = v[t][k];
         if (lane == 0) {
             s_mag[b][wave] = best;
             s_row[b][wave] = brow;
         }
         __syncthreads();
-        // 2. every wave reduces the 16 partials (lanes 0-15, one 16-lane row);
-        // the pivot row's owner publishes it
-        static_assert(LU_WAVES == 16, "the partials fill one 16-lane row");
-        double bm = lane < LU_WAVES ? s_mag[b][lane] : -1.0;
-        int br = lane < LU_WAVES ? s_row[b][lane] : 0x7fffffff;
+        // 2. every wave reduces the 16 candidates (lanes 0-15, one 16-lane row)
+        static_assert(LU_WAVES == 16, "the partials fit one 16-lane row");
+        double bm = lane < nwaves ? s_mag[b][lane] : -1.0;
+        int br = lane < nwaves ? s_row[b][lane] : 0x7fffffff;
         pivot_row_reduce(bm, br);
         br = __builtin_amdgcn_readlane(br, 0);
-        if (br >= m) br = cj;   // no candidate compared larger (NaN column): keep the diagonal row
-        const int p = br;
+        // no candidate compared larger (NaN column): keep the diagonal row
+        const bool found = br < m;
+        const int p = found ? br : cj;
+        const double *prow = found ? s_cand[b][((p - c0) % NT) >> 6] : s_crow[b];
         if (tid == 0) s_ipiv[j] = p + 1;
-#pragma unroll
-        for (int t = 0; t < T; ++t) {
-            const int r = c0 + tid + t * LU_THREADS;
-            if (r == p)
-#pragma unroll
-                for (int k = 0; k < W; ++k) s_prow[b][k] = v[t][k];
-        }
-        __syncthreads();
-    // 3. swap, scale the column below the diagonal, rank-1 update of the panel
-        const double piv = s_prow[b][j];
+        // 3. swap, scale the column below the diagonal, rank-1 update of the panel
+        const unsigned long long pbits = __builtin_bit_cast(unsigned long long, prow[j]);
+        const double piv = __builtin_bit_cast(
+            double, ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(pbits >> 32)) << 32) |
+                        (unsigned)__builtin_amdgcn_readfirstlane((int)pbits));
         if (tid == 0 && piv == 0.0 && zero_col < 0) zero_col = cj;
         const bool recip = fabs(piv) >= DBL_MIN;
         const double rpiv = 1.0 / piv;
 #pragma unroll
         for (int t = 0; t < T; ++t) {
-            const int r = c0 + tid + t * LU_THREADS;
-            if (r == cj)
-#pragma unroll
-                for (int k = 0; k < W; ++k) v[t][k] = s_prow[b][k];
-            else if (r == p)
-#pragma unroll
-                for (int k = 0; k < W; ++k) v[t][k] = s_crow[b][k];
-            if (r > cj && r < m && piv != 0.0) {
+            const int r = c0 + tid + t * NT;
+            // (the empty asm keeps each copy a register move: as a block copy
+            // from LDS the register row would be given memory)
+            if (t == 0 && r == cj) {
 #pragma unroll
                 for (int k = 0; k < W; ++k) {
-                    if (k == j) v[t][k] = recip ? v[t][k] * rpiv : v[t][k] / piv;
+                    double x = prow[k];
+                    asm volatile("" : "+v"(x));
+                    v[0][k] = x;
                 }
-                double l = 0.0;
+            } else if (r == p) {
 #pragma unroll
-                for (int k = 0; k < W; ++k)
-                    if (k == j) l = v[t][k];
-#pragma unroll
-                for (int k = 0; k < W; ++k)
-                    if (k > j && k < w) v[t][k] = fma(-l, s_prow[b][k], v[t][k]);
+                for (int k = 0; k < W; ++k) {
+                    double x = s_crow[b][k];
+                    asm volatile("" : "+v"(x));
+                    v[t][k] = x;
+                }
+            }
+            if ((t > 0 || r > cj) && live[t] && piv != 0.0) {
+                const double l = recip ? v[t][j] * rpiv : v[t][j] / piv;
+                v[t][j] = l;
+#pragma clang loop unroll(full)
+                for (int k = j + 1; k < W; ++k)
+                    if (incol(k)) v[t][k] = fma(-l, prow[k], v[t][k]);
             }
         }
     }
+    __syncthreads();   // s_ipiv is complete
 
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-        const int r = c0 + tid + t * LU_THREADS;
-        if (r < m)
+        const int r = c0 + tid + t * NT;
+        if (live[t])
 #pragma unroll
             for (int k = 0; k < W; ++k)
-                if (k < w) A[r + (int64_t)(c0 + k) * lda] = v[t][k];
+                if (incol(k)) A[r + (int64_t)(c0 + k) * lda] = v[t][k];
     }
     if (tid < w) ipiv[c0 + tid] = s_ipiv[tid];
     if (tid == 0) {
@@ -396,9 +422,15 @@ __global__ __launch_bounds__(64) void getrf_trsm_small_kernel(const double *__re
 // T > 0: the register leaf (T rows per thread); T == 0: the global-memory leaf.
 template <int T, int W>
 int leaf(double *A, int64_t lda, int m, int ncols, int c0, int w, int *ipiv, int *info, hipStream_t s) {
-    if constexpr (T > 0)
-        hipLaunchKernelGGL((getrf_leaf_kernel<T, W>), dim3(1), dim3(LU_THREADS), 0, s, A, lda, m, c0, w, ipiv, info);
-    else
+    if constexpr (T > 0) {
+        // as many waves as T rows per thread need (at least one)
+        const int per = (m - c0 + T - 1) / T;
+        const int nt = per <= 64 ? 64 : per >= LU_THREADS ? LU_THREADS : (per + 63) / 64 * 64;
+        if (w == W)
+            hipLaunchKernelGGL((getrf_leaf_kernel<T, W, true>), dim3(1), dim3(nt), 0, s, A, lda, m, c0, w, ipiv, info);
+        else
+            hipLaunchKernelGGL((getrf_leaf_kernel<T, W, false>), dim3(1), dim3(nt), 0, s, A, lda, m, c0, w, ipiv, info);
+    } else
         hipLaunchKernelGGL((getrf_leaf_global_kernel<W>), dim3(1), dim3(1024), 0, s, A, lda, m, c0, w, ipiv, info);
     if (w > 0 && ncols > w) {
         constexpr int cols_per_block = SWAP_THREADS / swap_lanes_per_column(W);
