@@ -43,12 +43,16 @@ struct GemmParams {
 // conflict-free LDS store (the transposing store of the [row][k] layout puts
 // 8 lanes on each FP64 bank pair, and writes bf16 two bytes at a time).  The
 // bf16 wave reads such a slab with LDS transpose reads (WaveBF16).
+// Row pitch padding (elements) of a k-major slab.
+template <typename T>
+constexpr int kmajor_pad() { return sizeof(T) == 2 ? 8 : 4; }
+
 template <typename T, int ROWS, int BK, int NT, bool KCONTIG, bool VEC, bool KMAJOR = false>
 struct TileLoader {
     using V = typename Vec16<T>::t;
     static constexpr int VW = Vec16<T>::N;
     static constexpr int PITCH = BK + VW;
-    static constexpr int KPITCH = ROWS + (sizeof(T) == 2 ? 8 : 4);
+    static constexpr int KPITCH = ROWS + kmajor_pad<T>();
     static constexpr int SIZE = KMAJOR ? BK * KPITCH : ROWS * PITCH;   // elements per stage
     static constexpr int NVEC = ROWS * BK / VW;
     static constexpr bool KGATHER = !KCONTIG && sizeof(T) == 4;

@@ -33,6 +33,15 @@ struct GroupDesc {
     double alpha, beta;
 };
 
+// Elements of one operand slab of ROWS rows: the larger of its layouts
+// ([row][k], and k-major for FP64 and bf16).
+template <typename TI, int ROWS, int BK>
+constexpr size_t slab_elems() {
+    const size_t rowk = (size_t)ROWS * (BK + 16 / sizeof(TI));
+    const size_t kmaj = sizeof(TI) == 4 ? 0 : (size_t)BK * (ROWS + kmajor_pad<TI>());
+    return rowk > kmaj ? rowk : kmaj;
+}
+
 // One BM x BN output tile of problem d; TI = operand element type, TO = C's.
 // CHUNKED problems (every dimension a multiple of the 16-byte chunk, aligned
 // operands) load whole chunks without branches and fetch slabs two ahead:
@@ -58,8 +67,8 @@ __device__ __forceinline__ void grouped_tile(const GroupDesc &d, int tm, int tn,
         BF, WaveBF16<WTM, WTN, BK, AKP, BKP>,
         std::conditional_t<DP, WaveF64<WTM, WTN, BK, AKP, BKP>, typename WaveSel<TI, WTM, WTN, BK>::type>>;
     constexpr int BOFF = LA::SIZE;               // B's slab follows A's in a stage
-    constexpr int STAGE = LA::SIZE + LB::SIZE;   // at most (BM + BN) * (BK + 16 bytes), what the launch reserves
-    static_assert(STAGE <= (BM + BN) * LA::PITCH, "stage exceeds the reserved LDS");
+    constexpr int STAGE = LA::SIZE + LB::SIZE;   // at most what the launch reserves per stage
+    static_assert(STAGE <= slab_elems<TI, BM, BK>() + slab_elems<TI, BN, BK>(), "stage exceeds the reserved LDS");
     TI *smem = reinterpret_cast<TI *>(smem_raw);
     const TI *A = static_cast<const TI *>(d.A);
     const TI *B = static_cast<const TI *>(d.B);
@@ -357,7 +366,7 @@ template <> struct GroupedCfg<uint16_t, float> { using S = TileCfg64x64BF16; usi
 
 template <typename TI, typename C>
 constexpr size_t stage_bytes() {
-    return 2ull * (C::BM + C::BN) * (C::BK + 16 / sizeof(TI)) * sizeof(TI);
+    return 2ull * (slab_elems<TI, C::BM, C::BK>() + slab_elems<TI, C::BN, C::BK>()) * sizeof(TI);
 }
 
 // LDS for the classes the plan uses (a plan with small tiles only keeps more
