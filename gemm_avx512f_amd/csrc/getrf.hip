@@ -24,66 +24,57 @@ namespace {
 constexpr int LU_THREADS = 1024;
 constexpr int LU_WAVES = LU_THREADS / 64;
 
-// Larger magnitude wins; equal magnitudes go to the smaller row (idamax takes
-// the first maximum).  A NaN magnitude never wins.
-__device__ inline bool pivot_better(double m1, int r1, double m2, int r2) {
-    return m1 > m2 || (m1 == m2 && r1 < r2);
-}
+// Pivot reductions in two passes: the largest magnitude first, then the
+// smallest row among the lanes that hold it.  That is the order (larger |a|,
+// then smaller row) in which idamax takes the first maximum, so the pivots
+// are those of one combined comparison; the passes take a plain maximum and
+// a plain minimum per step instead of a compare-and-select of two values.
+// Magnitudes are never NaN here (a NaN never becomes a candidate), and
+// lanes without a candidate hold -1 and row 0x7fffffff.
 
-// One step of a pivot reduction within 16-lane rows: the candidate of the
-// lane that the DPP pattern CTRL names (a data-parallel lane move, no LDS
-// round trip) against the lane's own.
+// max(x, the value of the lane that the DPP pattern CTRL names), within 16-lane rows
 template <int CTRL>
-__device__ __forceinline__ void pivot_dpp(double &best, int &brow) {
-    const long long bits = __double_as_longlong(best);
+__device__ __forceinline__ double dpp_max(double x) {
+    const long long bits = __double_as_longlong(x);
     const int lo = (int)bits, hi = (int)(bits >> 32);
     const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
     const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
-    const int orow = __builtin_amdgcn_update_dpp(brow, brow, CTRL, 0xF, 0xF, false);
-    const double om = __longlong_as_double((long long)(((unsigned long long)(unsigned)ohi << 32) | (unsigned)olo));
-    if (pivot_better(om, orow, best, brow)) {
-        best = om;
-        brow = orow;
-    }
+    return fmax(x, __longlong_as_double((long long)(((unsigned long long)(unsigned)ohi << 32) | (unsigned)olo)));
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_min(int x) {
+    return min(x, __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false));
+}
+// every lane of a 16-lane row: exchanges with lane ^ 1 and lane ^ 2 (quad
+// permutes), then with the mirrored lane in the half row and in the row
+__device__ __forceinline__ double row_max(double x) {
+    return dpp_max<0x140>(dpp_max<0x141>(dpp_max<0x4E>(dpp_max<0xB1>(x))));
+}
+__device__ __forceinline__ int row_min(int x) {
+    return dpp_min<0x140>(dpp_min<0x141>(dpp_min<0x4E>(dpp_min<0xB1>(x))));
+}
+__device__ __forceinline__ double lane_double(double x, int l) {
+    const long long bits = __double_as_longlong(x);
+    return __longlong_as_double(
+        (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(bits >> 32), l) << 32) |
+                    (unsigned)__builtin_amdgcn_readlane((int)bits, l)));
 }
 
-// Every lane of a 16-lane row ends with the row's best candidate: exchanges
-// with lane ^ 1 and lane ^ 2 (quad permutes), then with the mirrored lane in
-// the half row and in the row, each pairing two halves that already hold
-// their own best.
+// Every lane of a 16-lane row ends with the row's best candidate.
 __device__ __forceinline__ void pivot_row_reduce(double &best, int &brow) {
-    pivot_dpp<0xB1>(best, brow);    // quad_perm [1, 0, 3, 2]
-    pivot_dpp<0x4E>(best, brow);    // quad_perm [2, 3, 0, 1]
-    pivot_dpp<0x141>(best, brow);   // row_half_mirror
-    pivot_dpp<0x140>(best, brow);   // row_mirror
+    const double m = row_max(best);
+    brow = row_min(best == m ? brow : 0x7fffffff);
+    best = m;
 }
 
-// The wave's best candidate (every lane gets it): the four rows' results,
-// read as wave-uniform values.  The order on (|a|, row) is total (NaN
-// magnitudes never become candidates), so the result does not depend on the
-// reduction's shape.
+// The wave's best candidate (every lane gets it, as wave-uniform values).
 __device__ __forceinline__ void pivot_wave_reduce(double &best, int &brow) {
-    pivot_row_reduce(best, brow);
-    const long long bits = __double_as_longlong(best);
-    const int lo = (int)bits, hi = (int)(bits >> 32);
-    double b = best;
-    int r = brow;
-    b = __longlong_as_double((long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, 0) << 32) |
-                                         (unsigned)__builtin_amdgcn_readlane(lo, 0)));
-    r = __builtin_amdgcn_readlane(brow, 0);
-#pragma unroll
-    for (int row = 1; row < 4; ++row) {
-        const double ob = __longlong_as_double(
-            (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, 16 * row) << 32) |
-                        (unsigned)__builtin_amdgcn_readlane(lo, 16 * row)));
-        const int orow = __builtin_amdgcn_readlane(brow, 16 * row);
-        if (pivot_better(ob, orow, b, r)) {
-            b = ob;
-            r = orow;
-        }
-    }
-    best = b;
-    brow = r;
+    const double rm = row_max(best);
+    const double m = fmax(fmax(lane_double(rm, 0), lane_double(rm, 16)), fmax(lane_double(rm, 32), lane_double(rm, 48)));
+    const int rr = row_min(best == m ? brow : 0x7fffffff);
+    brow = min(min(__builtin_amdgcn_readlane(rr, 0), __builtin_amdgcn_readlane(rr, 16)),
+               min(__builtin_amdgcn_readlane(rr, 32), __builtin_amdgcn_readlane(rr, 48)));
+    best = m;
 }
 
 // Factor columns [c0, c0 + w) of A over rows [c0, m) (T rows per thread,
