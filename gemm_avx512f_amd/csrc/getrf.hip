@@ -163,14 +163,17 @@ __global__ __launch_bounds__(LU_THREADS) void getrf_leaf_kernel(double *__restri
         __syncthreads();
         // 2. every wave reduces the 16 candidates (lanes 0-15, one 16-lane row)
         static_assert(LU_WAVES == 16, "the partials fit one 16-lane row");
+        const int wrow = lane < nwaves ? s_row[b][lane] : 0x7fffffff;   // lane w: wave w's candidate row
         double bm = lane < nwaves ? s_mag[b][lane] : -1.0;
-        int br = lane < nwaves ? s_row[b][lane] : 0x7fffffff;
+        int br = wrow;
         pivot_row_reduce(bm, br);
         br = __builtin_amdgcn_readlane(br, 0);
         // no candidate compared larger (NaN column): keep the diagonal row
         const bool found = br < m;
         const int p = found ? br : cj;
-        const double *prow = found ? s_cand[b][((p - c0) % NT) >> 6] : s_crow[b];
+        // the winning wave is the lane whose candidate row is the pivot row (rows are distinct)
+        const unsigned long long won = __builtin_amdgcn_ballot_w64(found && wrow == p);
+        const double *prow = found ? s_cand[b][__builtin_ctzll(won | (1ull << 63))] : s_crow[b];
         if (tid == 0) s_ipiv[j] = p + 1;
         // 3. swap, scale the column below the diagonal, rank-1 update of the panel
         const unsigned long long pbits = __builtin_bit_cast(unsigned long long, prow[j]);
@@ -183,21 +186,26 @@ __global__ __launch_bounds__(LU_THREADS) void getrf_leaf_kernel(double *__restri
 #pragma unroll
         for (int t = 0; t < T; ++t) {
             const int r = c0 + tid + t * NT;
-            // (the empty asm keeps each copy a register move: as a block copy
-            // from LDS the register row would be given memory)
+            // (all reads first, then an empty asm on each value: it keeps the
+            // copy register moves -- as a block copy from LDS the register
+            // row would be given memory -- without waiting on each read)
             if (t == 0 && r == cj) {
+                double x[W];
+#pragma unroll
+                for (int k = 0; k < W; ++k) x[k] = prow[k];
 #pragma unroll
                 for (int k = 0; k < W; ++k) {
-                    double x = prow[k];
-                    asm volatile("" : "+v"(x));
-                    v[0][k] = x;
+                    asm volatile("" : "+v"(x[k]));
+                    v[0][k] = x[k];
                 }
             } else if (r == p) {
+                double x[W];
+#pragma unroll
+                for (int k = 0; k < W; ++k) x[k] = s_crow[b][k];
 #pragma unroll
                 for (int k = 0; k < W; ++k) {
-                    double x = s_crow[b][k];
-                    asm volatile("" : "+v"(x));
-                    v[t][k] = x;
+                    asm volatile("" : "+v"(x[k]));
+                    v[t][k] = x[k];
                 }
             }
             if ((t > 0 || r > cj) && live[t] && piv != 0.0) {
