@@ -183,6 +183,13 @@ __global__ __launch_bounds__(LU_THREADS) void getrf_leaf_kernel(double *__restri
         if (tid == 0 && piv == 0.0 && zero_col < 0) zero_col = cj;
         const bool recip = fabs(piv) >= DBL_MIN;
         const double rpiv = 1.0 / piv;
+        // the pivot row's entries right of the diagonal, read once for all of
+        // the thread's register rows (read inside each row's branch they would
+        // be one LDS round trip per row)
+        double pk[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k)
+            if (T > 1 && k > j && incol(k)) pk[k] = prow[k];
 #pragma unroll
         for (int t = 0; t < T; ++t) {
             const int r = c0 + tid + t * NT;
@@ -213,7 +220,7 @@ __global__ __launch_bounds__(LU_THREADS) void getrf_leaf_kernel(double *__restri
                 v[t][j] = l;
 #pragma clang loop unroll(full)
                 for (int k = j + 1; k < W; ++k)
-                    if (incol(k)) v[t][k] = fma(-l, prow[k], v[t][k]);
+                    if (incol(k)) v[t][k] = fma(-l, T > 1 ? pk[k] : prow[k], v[t][k]);
             }
         }
     }
