@@ -35,26 +35,34 @@ __device__ __forceinline__ double opa(const double *A, int64_t lda, bool trans, 
     return trans ? A[c + r * lda] : A[r + c * lda];
 }
 
-// grid.x = diagonal block index, 256 threads per block.  All four waves load
-// the block into LDS at once (16 loads in flight per thread, one memory
-// latency); wave 0 then inverts it: lane c computes column c of the inverse
-// by substitution against e_c with the column in registers and the block's
-// rows read from LDS (every lane reads the same entry: a broadcast), in four
-// independent FMA chains per row; all four waves write it out.  W[b] (NB x
-// NB, ld NB) <- inv(op(A)_bb); rows/cols past the matrix edge are the
-// identity.  Block 0 also writes the NB zeros that follow the last block.
+// grid.x = diagonal block index, 256 threads per block.  W[b] (NB x NB, ld
+// NB) <- inv(op(A)_bb); rows/cols past the matrix edge are the identity.
+// Block 0 also writes the NB zeros that follow the last block.
+//
+// The block is inverted by doubling, with all four waves busy: the inverses
+// of its 16 diagonal 4x4 blocks by substitution, then for h = 4, 8, 16, 32
+// every pair of neighbouring h x h inverses becomes one 2h x 2h inverse,
+//     inv([[A, 0], [B, C]]) = [[inv(A), 0], [-inv(C) B inv(A), inv(C)]],
+// through Y = B inv(A) and -inv(C) Y.  The 16- and 32-row products run on
+// v_mfma_f64_16x16x4 (one 16x16 output block per wave), the smaller ones as
+// FMA sums, one output per thread.  An upper triangle is inverted as the
+// lower triangle of its index-reversed block (row r, column c at 63 - r,
+// 63 - c), whose inverse is the reversed inverse.  (Substituting one column
+// of the inverse per lane kept one wave issuing 2048 FMA instructions.)
 __global__ __launch_bounds__(256) void trsm_diag_inverse_kernel(const double *__restrict__ A, int64_t lda, int na,
                                                                 bool trans, bool oplower, bool unit,
                                                                 double *__restrict__ W) {
-    constexpr int NB = TRSM_NB, NT = 256;
-    // op(A) block [row][col], later the inverse; the pitch keeps rows 16-byte
-    // aligned for paired broadcast reads and spreads column accesses over banks
-    __shared__ __align__(16) double T[NB][NB + 2];
+    constexpr int NB = TRSM_NB, NT = 256, P = NB + 2;
+    __shared__ __align__(16) double T[NB][P];    // the (reversed, if upper) block: strict lower part
+    __shared__ __align__(16) double X[NB][P];    // its inverse (zero above the diagonal)
+    __shared__ __align__(16) double Y[NB][P];    // products B inv(A) of the current step
     __shared__ double piv[NB];
     const int b = blockIdx.x;
     const int64_t base = (int64_t)b * NB;
     const int nb = min(NB, na - (int)base);
-    const int tid = threadIdx.x, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // position of op(A)'s (r, c) in the lower-triangular working block
+    auto wr = [&](int r) { return oplower ? r : NB - 1 - r; };
     {
         // element e = tid + NT u: its contiguous index (row for op(A) = A, column
         // for op(A) = A^T) is the lane, so every load is coalesced
@@ -70,47 +78,86 @@ __global__ __launch_bounds__(256) void trsm_diag_inverse_kernel(const double *__
         for (int u = 0; u < NB * NB / NT; ++u) {
             const int o = (tid + NT * u) >> 6;
             const int r = trans ? o : lane, col = trans ? lane : o;
-            T[r][col] = v[u];
+            T[wr(r)][wr(col)] = v[u];
+            X[r][col] = 0.0;
         }
     }
     if (tid < NB) {
         double d = 1.0;
         if (tid < nb && !unit) d = opa(A, lda, trans, base + tid, base + tid);
-        piv[tid] = 1.0 / d;
+        piv[wr(tid)] = 1.0 / d;
     }
     __syncthreads();
-    if (tid < 64) {   // wave 0 (no barrier inside)
-        const int c = lane;
-        double x[NB];
-        if (oplower) {
+    // 4x4 diagonal blocks: thread (q, cc) substitutes column cc of block q
+    if (tid < NB) {
+        const int q = tid >> 2, cc = tid & 3, o = 4 * q;
+        double x[4];
 #pragma unroll
-            for (int i = 0; i < NB; ++i) {
-                double s[4] = {(i == c) ? 1.0 : 0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < 4; ++i) {
+            double s = (i == cc) ? 1.0 : 0.0;
 #pragma unroll
-                for (int j = 0; j < i; ++j) s[j & 3] = fma(-T[i][j], x[j], s[j & 3]);
-                x[i] = ((s[0] + s[1]) + (s[2] + s[3])) * piv[i];
-            }
-        } else {
-#pragma unroll
-            for (int i = NB - 1; i >= 0; --i) {
-                double s[4] = {(i == c) ? 1.0 : 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int j = i + 1; j < NB; ++j) s[(j - i - 1) & 3] = fma(-T[i][j], x[j], s[(j - i - 1) & 3]);
-                x[i] = ((s[0] + s[1]) + (s[2] + s[3])) * piv[i];
-            }
+            for (int j = 0; j < i; ++j) s = fma(-T[o + i][o + j], x[j], s);
+            x[i] = s * piv[o + i];
         }
-        // T is read by every lane of this wave only: its writes follow its reads
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int r = 0; r < NB; ++r) T[r][c] = x[r];
+        for (int i = 0; i < 4; ++i) X[o + i][o + cc] = x[i];
     }
     __syncthreads();
+    // h = 4, 8: pairs (top = 2h p) of h x h blocks; one output per thread
+#pragma unroll
+    for (int h = 4; h <= 8; h *= 2) {
+        const int outs = (NB / (2 * h)) * h * h;   // 128, 256
+        const int p = tid / (h * h), i = (tid / h) % h, j = tid % h, top = 2 * h * p;
+        if (tid < outs) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < h; ++k) s = fma(T[top + h + i][top + k], X[top + k][top + j], s);
+            Y[top + h + i][top + j] = s;
+        }
+        __syncthreads();
+        if (tid < outs) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < h; ++k) s = fma(X[top + h + i][top + h + k], Y[top + h + k][top + j], s);
+            X[top + h + i][top + j] = -s;
+        }
+        __syncthreads();
+    }
+    // h = 16, 32 on the matrix cores: D[i][j] = sum_k P[pr + i][pk + k] Q[qk + k][qc + j]
+    // (A[i][k] at lane i + 16k, B[k][j] at lane j + 16k, D[i][j] at lane
+    // j + 16 (i % 4), element i / 4)
+    auto block16 = [&](const double (*Pm)[P], int pr, int pk, const double (*Qm)[P], int qk, int qc, int K) {
+        d4_t acc = {0.0, 0.0, 0.0, 0.0};
+        const int i = lane & 15, kk = lane >> 4;
+        for (int k0 = 0; k0 < K; k0 += 4)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Pm[pr + i][pk + k0 + kk], Qm[qk + k0 + kk][qc + i], acc, 0, 0,
+                                                        0);
+        return acc;
+    };
+    auto put16 = [&](double (*D)[P], int dr, int dc, d4_t acc, bool neg) {
+        const int j = lane & 15, i0 = lane >> 4;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) D[dr + 4 * e + i0][dc + j] = neg ? -acc[e] : acc[e];
+    };
+    {   // h = 16: pairs at top 0 and 32, waves 0 and 1
+        const int top = 32 * wave;
+        if (wave < 2) put16(Y, top + 16, top, block16(T, top + 16, top, X, top, top, 16), false);
+        __syncthreads();
+        if (wave < 2) put16(X, top + 16, top, block16(X, top + 16, top + 16, Y, top + 16, top, 16), true);
+        __syncthreads();
+    }
+    {   // h = 32: one pair, four 16x16 output blocks, one per wave
+        const int bi = 16 * (wave >> 1), bj = 16 * (wave & 1);
+        put16(Y, 32 + bi, bj, block16(T, 32 + bi, 0, X, 0, bj, 32), false);
+        __syncthreads();
+        put16(X, 32 + bi, bj, block16(X, 32 + bi, 32, Y, 32, bj, 32), true);
+        __syncthreads();
+    }
     double *out = W + (int64_t)b * NB * NB;
 #pragma unroll
     for (int u = 0; u < NB * NB / NT; ++u) {
-        const int e = tid + NT * u;
-        out[e] = T[e & 63][e >> 6];
+        const int e = tid + NT * u, r = e & 63, c = e >> 6;
+        out[e] = X[wr(r)][wr(c)];
     }
     if (b == 0 && tid < NB) W[(int64_t)gridDim.x * NB * NB + tid] = 0.0;   // the zero operand of the panel kernel
 }
