@@ -5,9 +5,10 @@
 // and the Schur-complement GEMM, then the right half), so all but the leaf
 // columns run in the DTRSM and FP64 matrix-core GEMM kernels of this library.
 // A leaf is a panel of up to 32 columns and every row below its diagonal,
-// factored by one 1024-thread workgroup with the panel held in registers
-// (rows strided over threads); panels of more than 16384 rows stay in global
-// memory instead (getrf_leaf_global_kernel), so m is not limited.  After each leaf a chip-wide kernel applies
+// factored by one workgroup of up to 1024 threads with the panel held in
+// registers (rows strided over threads); panels of more than 16384 rows stay
+// in global memory instead (getrf_leaf_global_kernel), so m is not limited.
+// After each leaf a chip-wide kernel applies
 // the leaf's row interchanges to every other column of the matrix, so the
 // state after every leaf is that of LAPACK's unblocked dgetf2 at the same
 // column and no deferred row-swap passes are needed.
@@ -92,11 +93,12 @@ __device__ __forceinline__ void pivot_wave_reduce(double &best, int &brow) {
 // slot, so no second exchange is needed.  Slots are double buffered by
 // column parity: column j+2 rewrites column j's slots only after every wave
 // has passed column j+1's barrier, that is, after it finished reading them.
-// The column step is bound by instruction issue (up to 16 waves share the
-// CU's four SIMDs and each repeats the reductions), so the workgroup has no
-// more waves than the panel's rows need, conditions that hold for every row
-// but the first few (t > 0 rows lie below any diagonal of the leaf) are
-// spelled out, and a full-width leaf tests no column bounds.
+// A column step costs the length of one wave's dependent instruction chain
+// (every wave runs the whole step), so the step is kept short: conditions
+// that hold for every row but the first few (t > 0 rows lie below any
+// diagonal of the leaf) are spelled out, a full-width leaf tests no column
+// bounds, LDS reads are issued together before their values are used, and
+// the workgroup has no more waves than the panel's rows need.
 template <int T, int W, bool FULL>
 __global__ __launch_bounds__(LU_THREADS) void getrf_leaf_kernel(double *__restrict__ A, int64_t lda, int m, int c0,
                                                                 int w, int *__restrict__ ipiv,
