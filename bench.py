@@ -370,7 +370,7 @@ def grouped_leg(replays=20):
     out["zgemm3m_batch"] = {"problems": 128, "ms_per_call": round(ms, 3),
                             "tflops_complex_equiv": round(zflops / (ms / 1e3) / 1e12, 2),
                             "tflops_real_executed": round(0.75 * zflops / (ms / 1e3) / 1e12, 2),
-                            "note": "host-timed (each call builds its plan and synchronises)"}
+                            "note": "host-timed, each call synchronises; the warm-up call builds the plan, the timed calls (same arguments) reuse it"}
     return out
 
 

@@ -5,6 +5,9 @@
 #include "gemm3m.hpp"
 #include "../../include/gemm_amd.h"
 #include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,6 +29,37 @@ int check_all(const char *name, int count, const char *ta, const char *tb, const
         }
     }
     return 0;
+}
+
+// Plans of recent zgemm3m batches: the grouped plan of the 3·count real
+// products and the split / combine job tables, keyed by every argument and by
+// the workspace address the planes and products live at.  A batch repeated
+// with the same arguments (shapes, pointers, scalars) launches without
+// building a plan or uploading tables.  An entry is freed when the last call
+// holding it has synchronised and it has left the cache.
+struct Batch3m {
+    std::vector<uint64_t> key;
+    void *plan = nullptr;
+    SplitJob3m *d_split = nullptr;
+    CombineJob3m *d_comb = nullptr;
+    ~Batch3m() {
+        if (plan) grouped_plan_destroy(plan);
+        if (d_split) (void)hipFree(d_split);
+        if (d_comb) (void)hipFree(d_comb);
+    }
+};
+constexpr size_t BATCH3M_CACHED = 4;
+std::mutex g_b3_mu;
+// least recently used first; never destroyed (no device frees after the HIP
+// runtime has shut down at process exit)
+std::vector<std::shared_ptr<Batch3m>> &g_b3 = *new std::vector<std::shared_ptr<Batch3m>>();
+
+template <typename T>
+uint64_t key_bits(T v) {
+    uint64_t u = 0;
+    static_assert(sizeof(T) <= sizeof(u), "key words are 64-bit");
+    std::memcpy(&u, &v, sizeof(T));
+    return u;
 }
 
 }  // namespace
@@ -115,16 +149,45 @@ int gemm_amd_zgemm3m_batched(int count, const char *transa, const char *transb, 
         const size_t mp = even(m[p]), np = even(n[p]), kp = even(k[p]);
         off[p + 1] = off[p] + 3 * (mp * kp + kp * np + mp * np);
     }
-    // then the split and combine job tables
     const size_t tab = (off[count] * sizeof(double) + 255) / 256 * 256;
     const size_t tabs = 2 * (size_t)count * sizeof(SplitJob3m), tabc = (size_t)count * sizeof(CombineJob3m);
     int st = 0;
     WsLease lease(s);
-    char *wsb = (char *)lease.get(WS_COMPLEX, tab + tabs + tabc + 256, &st);
+    char *wsb = (char *)lease.get(WS_COMPLEX, tab + 256, &st);
     if (st) return st;
     double *ws = (double *)wsb;
-    SplitJob3m *d_split = (SplitJob3m *)(wsb + tab);
-    CombineJob3m *d_comb = (CombineJob3m *)(wsb + tab + tabs);
+    std::vector<uint64_t> key;
+    key.reserve(2 + 16 * (size_t)count);
+    key.push_back((uint64_t)count);
+    key.push_back(key_bits(wsb));
+    for (int p = 0; p < count; ++p) {
+        for (uint64_t v : {(uint64_t)(unsigned char)transa[p], (uint64_t)(unsigned char)transb[p], (uint64_t)m[p],
+                           (uint64_t)n[p], (uint64_t)k[p], key_bits(alpha[2 * p]), key_bits(alpha[2 * p + 1]),
+                           key_bits(a[p]), (uint64_t)lda[p], key_bits(b[p]), (uint64_t)ldb[p], key_bits(beta[2 * p]),
+                           key_bits(beta[2 * p + 1]), key_bits(c[p]), (uint64_t)ldc[p]})
+            key.push_back(v);
+    }
+    std::shared_ptr<Batch3m> bp;
+    {
+        std::lock_guard<std::mutex> lk(g_b3_mu);
+        for (size_t i = 0; i < g_b3.size(); ++i)
+            if (g_b3[i]->key == key) {
+                bp = g_b3[i];
+                g_b3.erase(g_b3.begin() + (long)i);
+                g_b3.push_back(bp);   // most recently used
+                break;
+            }
+    }
+    if (bp) {
+        int e = launch_gemm3m_split_batch_d(bp->d_split, 2 * count, s);
+        if (!e) e = grouped_plan_launch(bp->plan, s);
+        if (!e) e = launch_gemm3m_combine_batch_d(bp->d_comb, count, s);
+        const hipError_t se = hipStreamSynchronize(s);
+        if (e) return hip_status((hipError_t)e, "ZGEMM3M_BATCHED");
+        return hip_status(se, "ZGEMM3M_BATCHED");
+    }
+    bp = std::make_shared<Batch3m>();
+    bp->key = std::move(key);
     std::vector<SplitJob3m> sj(2 * (size_t)count);
     std::vector<CombineJob3m> cj(count);
     std::vector<char> ta3, tb3;
@@ -158,18 +221,28 @@ int gemm_amd_zgemm3m_batched(int count, const char *transa, const char *transb, 
             C3.push_back(pt + q * (size_t)mp * np);
         }
     }
-    void *plan = grouped_plan_create(GROUPED_F64, 3 * count, ta3.data(), tb3.data(), m3.data(), n3.data(), k3.data(),
-                                     al3.data(), A3.data(), la3.data(), B3.data(), lb3.data(), be3.data(), C3.data(),
-                                     lc3.data(), rt().cus, 0, &st);
-    if (!plan) return hip_status((hipError_t)st, "ZGEMM3M_BATCHED plan");
-    // the host tables stay alive until the synchronisation below
-    int e = (int)hipMemcpyAsync(d_split, sj.data(), tabs, hipMemcpyHostToDevice, s);
-    if (!e) e = (int)hipMemcpyAsync(d_comb, cj.data(), tabc, hipMemcpyHostToDevice, s);
-    if (!e) e = launch_gemm3m_split_batch_d(d_split, 2 * count, s);
-    if (!e) e = grouped_plan_launch(plan, s);
-    if (!e) e = launch_gemm3m_combine_batch_d(d_comb, count, s);
-    hipError_t se = hipStreamSynchronize(s);   // the plan's and the jobs' tables must outlive the launches
-    grouped_plan_destroy(plan);
+    bp->plan = grouped_plan_create(GROUPED_F64, 3 * count, ta3.data(), tb3.data(), m3.data(), n3.data(), k3.data(),
+                                   al3.data(), A3.data(), la3.data(), B3.data(), lb3.data(), be3.data(), C3.data(),
+                                   lc3.data(), rt().cus, 0, &st);
+    if (!bp->plan) return hip_status((hipError_t)st, "ZGEMM3M_BATCHED plan");
+    hipError_t he = hipMalloc(&bp->d_split, tabs);
+    if (he == hipSuccess) he = hipMalloc(&bp->d_comb, tabc);
+    if (he == hipSuccess) he = hipMemcpy(bp->d_split, sj.data(), tabs, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(bp->d_comb, cj.data(), tabc, hipMemcpyHostToDevice);
+    if (he != hipSuccess) return hip_status(he, "ZGEMM3M_BATCHED tables");
+    int e = launch_gemm3m_split_batch_d(bp->d_split, 2 * count, s);
+    if (!e) e = grouped_plan_launch(bp->plan, s);
+    if (!e) e = launch_gemm3m_combine_batch_d(bp->d_comb, count, s);
+    const hipError_t se = hipStreamSynchronize(s);
+    if (!e && se == hipSuccess) {
+        std::shared_ptr<Batch3m> evicted;   // freed outside the lock
+        std::lock_guard<std::mutex> lk(g_b3_mu);
+        g_b3.push_back(bp);
+        if (g_b3.size() > BATCH3M_CACHED) {
+            evicted = g_b3.front();
+            g_b3.erase(g_b3.begin());
+        }
+    }
     if (e) return hip_status((hipError_t)e, "ZGEMM3M_BATCHED");
     return hip_status(se, "ZGEMM3M_BATCHED");
 }

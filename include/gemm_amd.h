@@ -294,7 +294,10 @@ int gemm_amd_grouped_destroy(void *plan);
 
 /* A batch of independent complex-double 3M products (zgemm3m_ contract per
  * problem; alpha/beta hold 2*count doubles): the three real products of every
- * problem run together in one grouped FP64 launch. */
+ * problem run together in one grouped FP64 launch.  Returns when the batch has
+ * run.  The library keeps the plans of the last 4 distinct argument lists, so
+ * repeating a batch with the same arguments (shapes, pointers, scalars)
+ * builds no new plan. */
 int gemm_amd_zgemm3m_batched(int count, const char *transa, const char *transb, const int *m, const int *n,
                              const int *k, const double *alpha, const double *const *a, const int *lda,
                              const double *const *b, const int *ldb, const double *beta, double *const *c,

@@ -372,6 +372,28 @@ def test_zgemm3m_batched_vs_oracle():
         assert err <= 8 * (k + 4) * 2.0 ** -53, (p["transa"], p["transb"], m, n, k, err)
 
 
+def test_zgemm3m_batched_repeat_and_changed_scalar():
+    """A repeated batch (the library reuses its plan) gives the same bits; a
+    changed alpha is not served by the old plan (alpha doubled doubles C)."""
+    P = zcase(7, 21, 1, 90, [("N", "N"), ("T", "C"), ("C", "N")])
+    for p in P:
+        p["beta"] = 0j
+    zgemm3m_batched(P)
+    synchronize()
+    first = [p["c"].to_host_padded().copy() for p in P]
+    zgemm3m_batched(P)
+    synchronize()
+    for p, f in zip(P, first):
+        assert np.array_equal(p["c"].to_host_padded(), f)
+    P[0]["alpha"] = 2 * P[0]["alpha"]
+    zgemm3m_batched(P)
+    synchronize()
+    m = P[0]["m"]
+    assert np.array_equal(P[0]["c"].to_host_padded()[:m], 2 * first[0][:m])
+    for p, f in zip(P[1:], first[1:]):
+        assert np.array_equal(p["c"].to_host_padded(), f)
+
+
 def test_zgemm3m_batch_128_matches_single():
     """128 complex-double problems of size 256..1024 against one zgemm3m call per problem."""
     rng = np.random.default_rng(128)
