@@ -25,7 +25,12 @@ namespace gemm_amd {
 
 
 // ---------------------------------------------------------------- kernel
-template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, bool VEC>
+// FEPI (FP64 only): tiles wholly inside C store through store_full (no range
+// checks, no 64-bit multiply per element).  It is a separate instance, taken
+// for shallow K where the epilogue is a visible part of a tile, because the
+// extra path changes how the compiler lays out the main loop of the others
+// (the 8192^3 launch measured 2% slower with it).  Both compute the same bits.
+template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, bool VEC, bool FEPI = false>
 // FP64 wave tiles of up to 64x32 fit two waves per SIMD without spilling;
 // 64x64 ones need the whole register file of one.
 __global__ __launch_bounds__(WM *WN * 64, (sizeof(T) == 8 && (BM / WM) * (BN / WN) <= 2048) ? 2 : 1)
@@ -163,15 +168,25 @@ void gemm_mfma_kernel(GemmParams<T> p) {
         }
     }
 
+    // FEPI: a tile wholly inside C stores without range checks (store_full);
+    // the partials of split-K are alpha = 1 stores of the same kind
+    const bool full = FEPI && mlim >= BM && nlim >= BN;
     if (p.work) {
         T *out = p.work + (int64_t)blockIdx.y * p.M * p.N;
-        w.for_each(lane, wm0, wn0, [&](int r, int c, T v) {
-            if (r < mlim && c < nlim) out[(m0 + r) + (int64_t)(n0 + c) * p.M] = v;
-        });
+        if constexpr (FEPI) {
+            if (full) w.store_full(out + m0 + (int64_t)n0 * p.M, p.M, lane, wm0, wn0, 1.0, 0.0);
+        }
+        if (!full) {
+            w.for_each(lane, wm0, wn0, [&](int r, int c, T v) {
+                if (r < mlim && c < nlim) out[(m0 + r) + (int64_t)(n0 + c) * p.M] = v;
+            });
+        }
     } else {
         T *cbase = p.C + m0 + (int64_t)n0 * p.ldc;
         const T alpha = p.alpha, beta = p.beta;
-        if (beta == T(0)) {
+        if (full) {
+            if constexpr (FEPI) w.store_full(cbase, p.ldc, lane, wm0, wn0, alpha, beta);
+        } else if (beta == T(0)) {
             w.for_each(lane, wm0, wn0, [&](int r, int c, T v) {
                 if (r < mlim && c < nlim) cbase[r + (int64_t)c * p.ldc] = alpha * v;
             });
@@ -250,14 +265,24 @@ size_t gemm_workspace_bytes(int m, int n, int, const GemmConfig &cfg, int elem_b
     return (size_t)cfg.splitk * (size_t)m * (size_t)n * (size_t)elem_bytes;
 }
 
-template <typename T, bool TA, bool TB, int BM, int BN, int WM, int WN, int BK, bool VEC>
-static hipError_t launch_one(const GemmParams<T> &p, int splits, hipStream_t s) {
+template <typename T, bool TA, bool TB, int BM, int BN, int WM, int WN, int BK, bool VEC, bool FEPI>
+static hipError_t launch_one_as(const GemmParams<T> &p, int splits, hipStream_t s) {
     constexpr size_t lds = 2ull * SlabLayout<T, TA, TB, BM, BN, BK>::STAGE * sizeof(T);
-    auto kern = gemm_mfma_kernel<T, TA, TB, BM, BN, BK, WM, WN, VEC>;
-    if (hipError_t e = lds_limit_once<gemm_mfma_kernel<T, TA, TB, BM, BN, BK, WM, WN, VEC>>(lds)) return e;
+    auto kern = gemm_mfma_kernel<T, TA, TB, BM, BN, BK, WM, WN, VEC, FEPI>;
+    if (hipError_t e = lds_limit_once<gemm_mfma_kernel<T, TA, TB, BM, BN, BK, WM, WN, VEC, FEPI>>(lds)) return e;
     dim3 grid(p.tiles_m * p.tiles_n, splits);
     hipLaunchKernelGGL(kern, grid, dim3(WM * WN * 64), lds, s, p);
     return hipGetLastError();
+}
+
+// FP64 launches of at most GEMM_FEPI_MAX_K per partition take the
+// fast-epilogue instance
+constexpr int GEMM_FEPI_MAX_K = 2048;
+template <typename T, bool TA, bool TB, int BM, int BN, int WM, int WN, int BK, bool VEC>
+static hipError_t launch_one(const GemmParams<T> &p, int splits, hipStream_t s) {
+    if constexpr (sizeof(T) == 8)
+        if (p.kchunk <= GEMM_FEPI_MAX_K) return launch_one_as<T, TA, TB, BM, BN, WM, WN, BK, VEC, true>(p, splits, s);
+    return launch_one_as<T, TA, TB, BM, BN, WM, WN, BK, VEC, false>(p, splits, s);
 }
 
 template <typename T, bool TA, bool TB, bool VEC>

@@ -315,6 +315,44 @@ struct WaveF64 {
                               : wn0 + sn * 16 + 4 * ((b + r) & 3) + j,
                           acc[sm][sn][r]);
     }
+
+    // C = alpha * acc (beta == 0: C is not read) or axpby(alpha, acc, beta, C)
+    // for a tile wholly inside C, c at the tile's origin: the map of for_each
+    // with one column pointer per (sub-tile column, rotation), formed once,
+    // and the rows as small offsets from it -- no range checks and no 64-bit
+    // multiply per element.
+    template <typename P>
+    __device__ __forceinline__ void store_full(P c, int64_t ldc, int lane, int wm0, int wn0, double alpha,
+                                               double beta) const {
+        const int i = lane >> 4, b = (lane >> 2) & 3, j = lane & 3;
+        int roff[SM];
+#pragma unroll
+        for (int sm = 0; sm < SM; ++sm)
+            roff[sm] = APK ? wm0 + 32 * (sm >> 1) + 2 * (4 * b + i) + (sm & 1) : wm0 + sm * 16 + 4 * b + i;
+        auto col = [&](int sn, int r) {
+            return BPK ? wn0 + 32 * (sn >> 1) + 2 * (4 * ((b + r) & 3) + j) + (sn & 1)
+                       : wn0 + sn * 16 + 4 * ((b + r) & 3) + j;
+        };
+        if (beta == 0.0) {
+#pragma unroll
+            for (int sn = 0; sn < SN; ++sn)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const P cp = c + (int64_t)col(sn, r) * ldc;
+#pragma unroll
+                    for (int sm = 0; sm < SM; ++sm) cp[roff[sm]] = alpha * acc[sm][sn][r];
+                }
+        } else {
+#pragma unroll
+            for (int sn = 0; sn < SN; ++sn)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const P cp = c + (int64_t)col(sn, r) * ldc;
+#pragma unroll
+                    for (int sm = 0; sm < SM; ++sm) cp[roff[sm]] = axpby(alpha, acc[sm][sn][r], beta, cp + roff[sm]);
+                }
+        }
+    }
 };
 
 // FP32: v_mfma_f32_32x32x2_f32.  Lane group g = lane/32 owns k = g*BK/2 ..

@@ -178,7 +178,12 @@ __device__ __forceinline__ void grouped_tile(const GroupDesc &d, int tm, int tn,
     const TO alpha = (TO)d.alpha, beta = (TO)d.beta;
     // beta is tested once per tile (its descriptor value is uniform); a tile
     // wholly inside C with beta == 0 stores without range checks
-    if constexpr (!DP) {
+    if constexpr (DP) {
+        if (mlim >= BM && nlim >= BN) {
+            w.store_full(cbase, d.ldc, lane, wm0, wn0, alpha, beta);
+            return;
+        }
+    } else {
         if (d.beta == 0.0 && mlim >= BM && nlim >= BN) {
             w.store_full(cbase, d.ldc, lane, wm0, wn0, alpha);
             return;
@@ -408,54 +413,83 @@ void *grouped_plan_create(int dtype, int count, const char *ta, const char *tb, 
     *status = 0;
     GroupedPlan *P = new GroupedPlan();
     P->dtype = dtype;
-    P->count = count;
-    std::vector<int> order(count);
-    std::iota(order.begin(), order.end(), 0);
-    // largest problems first: their tiles are spread over all CUs before the small ones
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        return (double)m[x] * n[x] * k[x] > (double)m[y] * n[y] * k[y];
+    if (large_min <= 0) large_min = grouped_default_large_min(dtype);
+    const int eb = dtype == GROUPED_F64 ? 8 : (dtype == GROUPED_F32 ? 4 : 2);   // operand element bytes
+    const int ebc = dtype == GROUPED_F64 ? 8 : 4;                                // C element bytes
+    // Work items: a problem on the large tile class whose last tile row would
+    // hold at most a small tile's rows runs as two items, the rows of whole
+    // large tiles and the remaining rows on small tiles (the classes compute
+    // the same bits), so that no large tile row is mostly padding.
+    struct Item {
+        int p, r0, rows, cls;
+    };
+    std::vector<Item> items;
+    items.reserve(count);
+    {
+        int BML, BNL, BMS, BNS;
+        tile_dims_of(dtype, 1, &BML, &BNL);
+        tile_dims_of(dtype, 0, &BMS, &BNS);
+        for (int p = 0; p < count; ++p) {
+            const int cls = m[p] >= large_min && n[p] >= (dtype == GROUPED_F64 ? large_min / 2 : large_min);
+            const int rem = m[p] % BML;
+            if (cls && m[p] > BML && rem > 0 && rem <= BMS) {
+                items.push_back({p, 0, m[p] - rem, 1});
+                items.push_back({p, m[p] - rem, rem, 0});
+            } else {
+                items.push_back({p, 0, m[p], cls});
+            }
+        }
+    }
+    const int nitems = (int)items.size();
+    P->count = nitems;
+    // largest items first: their tiles are spread over all CUs before the small ones
+    std::stable_sort(items.begin(), items.end(), [&](const Item &x, const Item &y) {
+        return (double)x.rows * n[x.p] * k[x.p] > (double)y.rows * n[y.p] * k[y.p];
     });
-    std::vector<GroupDesc> h(count);
+    std::vector<GroupDesc> h(nitems);
     int64_t base = 0;
     bool any_large = false;
-    if (large_min <= 0) large_min = grouped_default_large_min(dtype);
-    for (int i = 0; i < count; ++i) {
-        const int p = order[i];
+    for (int i = 0; i < nitems; ++i) {
+        const int p = items[i].p;
         GroupDesc &d = h[i];
-        d.m = m[p];
+        d.m = items[i].rows;
         d.n = n[p];
         d.k = k[p];
         d.ta = !(ta[p] == 'N' || ta[p] == 'n');
         d.tb = !(tb[p] == 'N' || tb[p] == 'n');
-        d.cls = m[p] >= large_min && n[p] >= (dtype == GROUPED_F64 ? large_min / 2 : large_min);
+        d.cls = items[i].cls;
         any_large = any_large || d.cls;
+        // the item's rows of op(A) and C
+        const int64_t r0 = items[i].r0;
+        const void *Ai = static_cast<const char *>(A[p]) + (d.ta ? r0 * lda[p] : r0) * eb;
+        void *Ci = static_cast<char *>(Cp[p]) + r0 * ebc;
         int BM, BN;
         tile_dims_of(dtype, d.cls, &BM, &BN);
-        d.tiles_m = (m[p] + BM - 1) / BM;
-        d.tiles_n = (n[p] + BN - 1) / BN;
+        d.tiles_m = (d.m + BM - 1) / BM;
+        d.tiles_n = (d.n + BN - 1) / BN;
         {
-            const int eb = dtype == GROUPED_F64 ? 8 : (dtype == GROUPED_F32 ? 4 : 2);
             const int vw = 16 / eb;
-            d.vec = ((uintptr_t)A[p] % 16 == 0) && ((uintptr_t)B[p] % 16 == 0) && (lda[p] % vw == 0) &&
+            d.vec = ((uintptr_t)Ai % 16 == 0) && ((uintptr_t)B[p] % 16 == 0) && (lda[p] % vw == 0) &&
                     (ldb[p] % vw == 0);
             // extents in bytes, for the 32-bit buffer offsets of the chunked path
-            const int64_t ar = d.ta ? k[p] : m[p], ac = d.ta ? m[p] : k[p];
+            const int64_t ar = d.ta ? k[p] : d.m, ac = d.ta ? d.m : k[p];
             const int64_t br = d.tb ? n[p] : k[p], bc = d.tb ? k[p] : n[p];
             const int64_t ea = ((int64_t)lda[p] * (ac - 1) + ar) * eb, ebb = ((int64_t)ldb[p] * (bc - 1) + br) * eb;
-            d.chunked = d.vec && m[p] % vw == 0 && n[p] % vw == 0 && k[p] % vw == 0 && ea < (1ll << 31) - 64 &&
+            d.chunked = d.vec && d.m % vw == 0 && n[p] % vw == 0 && k[p] % vw == 0 && ea < (1ll << 31) - 64 &&
                         ebb < (1ll << 31) - 64;
         }
         d.lda = lda[p];
         d.ldb = ldb[p];
         d.ldc = ldc[p];
-        d.A = A[p];
+        d.A = Ai;
         d.B = B[p];
-        d.C = Cp[p];
+        d.C = Ci;
         d.alpha = alpha[p];
         d.beta = beta[p];
         d.tile_base = base;
         base += (int64_t)d.tiles_m * d.tiles_n;
     }
+    count = nitems;   // descriptors from here on
     P->total_tiles = base;
     // tile -> problem, one word per tile: a tile finds its descriptor with one
     // load instead of a binary search of dependent loads over the table
