@@ -416,10 +416,12 @@ void *grouped_plan_create(int dtype, int count, const char *ta, const char *tb, 
     if (large_min <= 0) large_min = grouped_default_large_min(dtype);
     const int eb = dtype == GROUPED_F64 ? 8 : (dtype == GROUPED_F32 ? 4 : 2);   // operand element bytes
     const int ebc = dtype == GROUPED_F64 ? 8 : 4;                                // C element bytes
-    // Work items: a problem on the large tile class whose last tile row would
-    // hold at most a small tile's rows runs as two items, the rows of whole
-    // large tiles and the remaining rows on small tiles (the classes compute
-    // the same bits), so that no large tile row is mostly padding.
+    // Work items: an FP64 problem on the large tile class (128x64) whose last
+    // tile row would hold at most a small tile's rows runs as two items, the
+    // rows of whole large tiles and the remaining rows on small tiles (the
+    // classes compute the same bits), so that no large tile row is mostly
+    // padding.  (For bf16 and FP32, whose small tiles are also narrower and
+    // much slower per element than their 128x128 ones, it did not pay.)
     struct Item {
         int p, r0, rows, cls;
     };
@@ -432,7 +434,7 @@ void *grouped_plan_create(int dtype, int count, const char *ta, const char *tb, 
         for (int p = 0; p < count; ++p) {
             const int cls = m[p] >= large_min && n[p] >= (dtype == GROUPED_F64 ? large_min / 2 : large_min);
             const int rem = m[p] % BML;
-            if (cls && m[p] > BML && rem > 0 && rem <= BMS) {
+            if (dtype == GROUPED_F64 && cls && m[p] > BML && rem > 0 && rem <= BMS) {
                 items.push_back({p, 0, m[p] - rem, 1});
                 items.push_back({p, m[p] - rem, rem, 0});
             } else {

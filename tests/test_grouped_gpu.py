@@ -41,8 +41,8 @@ def expected_tiles(dtype, problems, large_min=None):
         large = p["m"] >= t and p["n"] >= (t // 2 if dtype == GROUPED_F64 else t)
         bm, bn = LARGE_TILE[dtype] if large else SMALL_TILE
         rem = p["m"] % bm
-        if large and p["m"] > bm and 0 < rem <= SMALL_TILE[0]:
-            # the rows of whole large tiles, then the rest on small tiles
+        if dtype == GROUPED_F64 and large and p["m"] > bm and 0 < rem <= SMALL_TILE[0]:
+            # FP64: the rows of whole large tiles, then the rest on small tiles
             total += (p["m"] // bm) * -(-p["n"] // bn) + -(-rem // SMALL_TILE[0]) * -(-p["n"] // SMALL_TILE[1])
             continue
         total += -(-p["m"] // bm) * -(-p["n"] // bn)
