@@ -39,6 +39,7 @@ int gemm_device(const char *name, char ta, char tb, int m, int n, int k, T alpha
     GemmConfig cfg = force ? *force : pick_gemm_config(sizeof(T) == 8 ? DT_F64 : DT_F32, !notrans(ta),
                                                        !notrans(tb), m, n, k, lda - (notrans(ta) ? m : k),
                                                        ldb - (notrans(tb) ? k : n), rt().cus);
+    if (!force) cfg.kgroups = true;
     const size_t wbytes = gemm_workspace_bytes(m, n, k, cfg, (int)sizeof(T));
     if (!wbytes)
         return hip_status((hipError_t)launch<T>(!notrans(ta), !notrans(tb), m, n, k, alpha, a, lda, b, ldb, beta, c,
@@ -109,6 +110,7 @@ void gemm_host(const char *name, const char *transa, const char *transb, const i
         // the staged copies are packed (leading dimension = rows)
         GemmConfig cfg = pick_gemm_config(sizeof(T) == 8 ? DT_F64 : DT_F32, !notrans(ta), !notrans(tb), m, n, k, 0,
                                           0, R.cus);
+        cfg.kgroups = true;
         const size_t wbytes = gemm_workspace_bytes(m, n, k, cfg, (int)sizeof(T));
         if (wbytes && (rc = buf(BUF_WORK).ensure(wbytes))) {
             fprintf(stderr, "%s: %s\n", name, get_error());

@@ -20,6 +20,7 @@
 #include "common.hpp"
 #include "gemm_kernels.hpp"
 #include "gemm_tile.hpp"
+#include "runtime.hpp"
 
 namespace gemm_amd {
 
@@ -30,17 +31,26 @@ namespace gemm_amd {
 // for shallow K where the epilogue is a visible part of a tile, because the
 // extra path changes how the compiler lays out the main loop of the others
 // (the 8192^3 launch measured 2% slower with it).  Both compute the same bits.
-template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, bool VEC, bool FEPI = false>
+//
+// KW > 1 (FP32): KW groups of WM x WN waves share each BK-deep slab, group g
+// multiplying its k = g*BK/KW .. (g+1)*BK/KW - 1; at the end groups 1..KW-1
+// pass their accumulators through LDS and group 0 adds them in group order
+// and stores.  It is for grids of at most one workgroup per CU, where a
+// 4-wave workgroup leaves each SIMD one wave to hide its latencies with.
+template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, bool VEC, bool FEPI = false,
+          int KW = 1>
 // FP64 wave tiles of up to 64x32 fit two waves per SIMD without spilling;
 // 64x64 ones need the whole register file of one.
-__global__ __launch_bounds__(WM *WN * 64, (sizeof(T) == 8 && (BM / WM) * (BN / WN) <= 2048) ? 2 : 1)
+__global__ __launch_bounds__(WM *WN *KW * 64, (sizeof(T) == 8 && (BM / WM) * (BN / WN) <= 2048) ? 2 : 1)
 void gemm_mfma_kernel(GemmParams<T> p) {
-    constexpr int NT = WM * WN * 64;
+    static_assert(KW == 1 || (sizeof(T) == 4 && BK % (8 * KW) == 0), "k-grouped waves are FP32 only");
+    constexpr int NT = WM * WN * KW * 64;
     constexpr int WTM = BM / WM, WTN = BN / WN;
     using Lay = SlabLayout<T, TA, TB, BM, BN, BK>;
     using LA = TileLoader<T, BM, BK, NT, TA, VEC, Lay::KMA>;   // op(A) rows = M; k-contiguous iff A is transposed
     using LB = TileLoader<T, BN, BK, NT, !TB, VEC, Lay::KMB>;  // op(B) rows = N; k-contiguous iff B is not transposed
-    using Wave = typename Lay::template Wave<WTM, WTN>;
+    using Wave = std::conditional_t<KW == 1, typename Lay::template Wave<WTM, WTN>,
+                                    WaveF32<WTM, WTN, BK / KW, Lay::PITCH>>;
     constexpr int STAGE = Lay::STAGE;
     constexpr int BOFF = LA::SIZE;
 
@@ -48,7 +58,8 @@ void gemm_mfma_kernel(GemmParams<T> p) {
     T *smem = reinterpret_cast<T *>(smem_raw);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm0 = (wave % WM) * WTM, wn0 = (wave / WM) * WTN;
+    const int wmn = KW == 1 ? wave : wave % (WM * WN), kg = KW == 1 ? 0 : wave / (WM * WN);
+    const int wm0 = (wmn % WM) * WTM, wn0 = (wmn / WM) * WTN;
 
     int tm, tn;
     tile_coords(blockIdx.x, p.tiles_m, p.tiles_n, 8, tm, tn);
@@ -65,7 +76,8 @@ void gemm_mfma_kernel(GemmParams<T> p) {
     LA la;
     LB lb;
     Wave w;
-    w.init(lane, wm0, wn0);
+    if constexpr (KW == 1) w.init(lane, wm0, wn0);
+    else w.init(lane, wm0, wn0, kg * (BK / KW));
 
     const int nk = (kend - kbeg + BK - 1) / BK;
     // Interior tiles (the bulk of any large problem) stream whole aligned
@@ -168,6 +180,34 @@ void gemm_mfma_kernel(GemmParams<T> p) {
         }
     }
 
+    if constexpr (KW > 1) {
+        constexpr int PER_WAVE = Wave::SM * Wave::SN * 16 * 64;   // accumulators of one wave
+        static_assert((KW - 1) * WM * WN * PER_WAVE <= 2 * STAGE, "partials fit the slab buffers");
+        float *red = reinterpret_cast<float *>(smem_raw);
+        __syncthreads();   // every wave is done with the slabs
+        if (kg > 0) {
+            float *dst = red + ((kg - 1) * WM * WN + wmn) * PER_WAVE + lane;
+#pragma unroll
+            for (int sm = 0; sm < Wave::SM; ++sm)
+#pragma unroll
+                for (int sn = 0; sn < Wave::SN; ++sn)
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) dst[((sm * Wave::SN + sn) * 16 + e) * 64] = w.acc[sm][sn][e];
+        }
+        __syncthreads();
+        if (kg > 0) return;
+#pragma unroll
+        for (int g = 1; g < KW; ++g) {
+            const float *src = red + ((g - 1) * WM * WN + wmn) * PER_WAVE + lane;
+#pragma unroll
+            for (int sm = 0; sm < Wave::SM; ++sm)
+#pragma unroll
+                for (int sn = 0; sn < Wave::SN; ++sn)
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) w.acc[sm][sn][e] += src[((sm * Wave::SN + sn) * 16 + e) * 64];
+        }
+    }
+
     // FEPI: a tile wholly inside C stores without range checks (store_full);
     // the partials of split-K are alpha = 1 stores of the same kind
     const bool full = FEPI && mlim >= BM && nlim >= BN;
@@ -265,13 +305,13 @@ size_t gemm_workspace_bytes(int m, int n, int, const GemmConfig &cfg, int elem_b
     return (size_t)cfg.splitk * (size_t)m * (size_t)n * (size_t)elem_bytes;
 }
 
-template <typename T, bool TA, bool TB, int BM, int BN, int WM, int WN, int BK, bool VEC, bool FEPI>
+template <typename T, bool TA, bool TB, int BM, int BN, int WM, int WN, int BK, bool VEC, bool FEPI, int KW = 1>
 static hipError_t launch_one_as(const GemmParams<T> &p, int splits, hipStream_t s) {
     constexpr size_t lds = 2ull * SlabLayout<T, TA, TB, BM, BN, BK>::STAGE * sizeof(T);
-    auto kern = gemm_mfma_kernel<T, TA, TB, BM, BN, BK, WM, WN, VEC, FEPI>;
-    if (hipError_t e = lds_limit_once<gemm_mfma_kernel<T, TA, TB, BM, BN, BK, WM, WN, VEC, FEPI>>(lds)) return e;
+    auto kern = gemm_mfma_kernel<T, TA, TB, BM, BN, BK, WM, WN, VEC, FEPI, KW>;
+    if (hipError_t e = lds_limit_once<gemm_mfma_kernel<T, TA, TB, BM, BN, BK, WM, WN, VEC, FEPI, KW>>(lds)) return e;
     dim3 grid(p.tiles_m * p.tiles_n, splits);
-    hipLaunchKernelGGL(kern, grid, dim3(WM * WN * 64), lds, s, p);
+    hipLaunchKernelGGL(kern, grid, dim3(WM * WN * KW * 64), lds, s, p);
     return hipGetLastError();
 }
 
@@ -285,11 +325,23 @@ static hipError_t launch_one(const GemmParams<T> &p, int splits, hipStream_t s) 
     return launch_one_as<T, TA, TB, BM, BN, WM, WN, BK, VEC, false>(p, splits, s);
 }
 
+// FP32 64x64 grids of at most one workgroup per CU with partitions of at
+// least 512 k take the k-grouped instance (1024^3: 27.9 -> 24.6 us; shallower
+// partitions measured slower)
+constexpr int KGROUP_MIN_K = 512;
+
 template <typename T, bool TA, bool TB, bool VEC>
-static hipError_t launch_tile(int tile, const GemmParams<T> &p, int splits, hipStream_t s) {
+static hipError_t launch_tile(int tile, const GemmParams<T> &p, int splits, bool kgroups, hipStream_t s) {
     switch (tile) {
     case TILE_128x128: return launch_one<T, TA, TB, 128, 128, 2, 2, 32, VEC>(p, splits, s);
-    case TILE_64x64: return launch_one<T, TA, TB, 64, 64, 2, 2, sizeof(T) == 8 ? 16 : 32, VEC>(p, splits, s);
+    case TILE_64x64:
+        if constexpr (sizeof(T) == 4) {
+            // four k-groups of four waves on 128-deep slabs: 16 waves per CU
+            // instead of 4
+            if (kgroups && p.kchunk >= KGROUP_MIN_K && (int64_t)p.tiles_m * p.tiles_n * splits <= rt().cus)
+                return launch_one_as<T, TA, TB, 64, 64, 2, 2, 128, VEC, false, 4>(p, splits, s);
+        }
+        return launch_one<T, TA, TB, 64, 64, 2, 2, sizeof(T) == 8 ? 16 : 32, VEC>(p, splits, s);
     case TILE_128x64: return launch_one<T, TA, TB, 128, 64, 2, 2, sizeof(T) == 8 ? 16 : 32, VEC>(p, splits, s);
     case TILE_LARGE:
         if constexpr (sizeof(T) == 8) return launch_one<T, TA, TB, 256, 64, 4, 1, 16, VEC>(p, splits, s);
@@ -331,17 +383,17 @@ static hipError_t launch_gemm_t(bool ta, bool tb, int m, int n, int k, T alpha, 
     const int sel = (ta ? 2 : 0) | (tb ? 1 : 0);
     if (vec) {
         switch (sel) {
-        case 0: e = launch_tile<T, false, false, true>(tile, p, splits, s); break;
-        case 1: e = launch_tile<T, false, true, true>(tile, p, splits, s); break;
-        case 2: e = launch_tile<T, true, false, true>(tile, p, splits, s); break;
-        default: e = launch_tile<T, true, true, true>(tile, p, splits, s); break;
+        case 0: e = launch_tile<T, false, false, true>(tile, p, splits, cfg.kgroups, s); break;
+        case 1: e = launch_tile<T, false, true, true>(tile, p, splits, cfg.kgroups, s); break;
+        case 2: e = launch_tile<T, true, false, true>(tile, p, splits, cfg.kgroups, s); break;
+        default: e = launch_tile<T, true, true, true>(tile, p, splits, cfg.kgroups, s); break;
         }
     } else {
         switch (sel) {
-        case 0: e = launch_tile<T, false, false, false>(tile, p, splits, s); break;
-        case 1: e = launch_tile<T, false, true, false>(tile, p, splits, s); break;
-        case 2: e = launch_tile<T, true, false, false>(tile, p, splits, s); break;
-        default: e = launch_tile<T, true, true, false>(tile, p, splits, s); break;
+        case 0: e = launch_tile<T, false, false, false>(tile, p, splits, cfg.kgroups, s); break;
+        case 1: e = launch_tile<T, false, true, false>(tile, p, splits, cfg.kgroups, s); break;
+        case 2: e = launch_tile<T, true, false, false>(tile, p, splits, cfg.kgroups, s); break;
+        default: e = launch_tile<T, true, true, false>(tile, p, splits, cfg.kgroups, s); break;
         }
     }
     if (e != hipSuccess || splits == 1) return e;

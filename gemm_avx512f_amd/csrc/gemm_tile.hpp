@@ -356,21 +356,23 @@ struct WaveF64 {
 };
 
 // FP32: v_mfma_f32_32x32x2_f32.  Lane group g = lane/32 owns k = g*BK/2 ..
-// (g+1)*BK/2 - 1 of each slab, read 4 at a time.
-template <int WTM, int WTN, int BK>
+// (g+1)*BK/2 - 1 of the wave's BK-deep part of each slab, read 4 at a time.
+// KP is the slab's row pitch and koff the first k of the wave's part (a slab
+// deeper than BK shared by several waves along k).
+template <int WTM, int WTN, int BK, int KP = BK + 4>
 struct WaveF32 {
     static constexpr int SM = WTM / 32, SN = WTN / 32;
-    static constexpr int PITCH = BK + 4;
+    static constexpr int PITCH = KP;
     static_assert(BK % 8 == 0, "FP32 slab depth must be a multiple of 8");
     f16_t acc[SM][SN];
     int aoff[SM], boff[SN];
 
-    __device__ __forceinline__ void init(int lane, int wm0, int wn0) {
+    __device__ __forceinline__ void init(int lane, int wm0, int wn0, int koff = 0) {
         const int q = lane & 31, g = lane >> 5;
 #pragma unroll
-        for (int sm = 0; sm < SM; ++sm) aoff[sm] = (wm0 + sm * 32 + q) * PITCH + (BK / 2) * g;
+        for (int sm = 0; sm < SM; ++sm) aoff[sm] = (wm0 + sm * 32 + q) * PITCH + koff + (BK / 2) * g;
 #pragma unroll
-        for (int sn = 0; sn < SN; ++sn) boff[sn] = (wn0 + sn * 32 + q) * PITCH + (BK / 2) * g;
+        for (int sn = 0; sn < SN; ++sn) boff[sn] = (wn0 + sn * 32 + q) * PITCH + koff + (BK / 2) * g;
 #pragma unroll
         for (int sm = 0; sm < SM; ++sm)
 #pragma unroll

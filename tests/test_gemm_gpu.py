@@ -59,6 +59,28 @@ def test_gemm_splitk(dtype, splitk):
     check_vs_oracle(dtype, "N", "T", 70, 90, 1500, 1.25, -0.5, seed=splitk, tile=1, splitk=splitk)
 
 
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+def test_sgemm_k_grouped_waves(ta, tb):
+    """An FP32 64x64 grid of at most one workgroup per CU, with at least 512 k per partition, splits each
+    slab's k over four groups of waves when the library picks the configuration itself, and group 0 adds
+    the other groups' accumulators.  A sweep-table entry makes the library pick 64x64 without split-K
+    for these shapes: ragged edges, K not a multiple of the 128-deep slab, a single tile."""
+    from gemm_avx512f_amd import tuning
+    shapes = [(1000, 520, 640), (333, 1000, 1100), (64, 64, 512)]
+    shp = np.zeros(len(shapes), dtype=tuning.SHAPE_DTYPE)
+    ms = np.full((len(shapes), 21), -1.0, dtype=np.float32)
+    for i, (m, n, k) in enumerate(shapes):
+        shp[i] = (0, ta == "T", tb == "T", m, n, k, 0)
+        ms[i, tuning.candidate_index(1, 1)] = 1.0
+    tuning.table_set(shp, ms)
+    try:
+        for i, (m, n, k) in enumerate(shapes):
+            assert tuning.pick_config(0, ta == "T", tb == "T", m, n, k, 0, mode=2) == (1, 1)
+            check_vs_oracle(np.float32, ta, tb, m, n, k, 1.5, 0.5, seed=300 + i)
+    finally:
+        tuning.table_clear()
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["d", "s"])
 def test_gemm_unaligned_leading_dims(dtype):
     # odd leading dimensions force the scalar-load path
