@@ -17,8 +17,8 @@ enum TileId : int {
 struct GemmConfig {
     int tile = TILE_128x128;
     int splitk = 1;     // number of K partitions (>1: partial sums + reduction pass)
-    // FP32 64x64 tiles may split each slab's k over four groups of waves
-    // when the grid has at most one workgroup per CU (a different k order:
+    // 64x64 (and FP64 128x64) tiles may split each slab's k over groups of
+    // waves when the grid has at most one workgroup per CU (a different k order:
     // set only where the library chooses the configuration, so an explicit
     // tile and split-K always name one kernel)
     bool kgroups = false;

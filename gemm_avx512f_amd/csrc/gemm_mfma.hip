@@ -32,7 +32,7 @@ namespace gemm_amd {
 // extra path changes how the compiler lays out the main loop of the others
 // (the 8192^3 launch measured 2% slower with it).  Both compute the same bits.
 //
-// KW > 1 (FP32): KW groups of WM x WN waves share each BK-deep slab, group g
+// KW > 1: KW groups of WM x WN waves share each BK-deep slab, group g
 // multiplying its k = g*BK/KW .. (g+1)*BK/KW - 1; at the end groups 1..KW-1
 // pass their accumulators through LDS and group 0 adds them in group order
 // and stores.  It is for grids of at most one workgroup per CU, where a
@@ -41,16 +41,18 @@ template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WM, int WN, 
           int KW = 1>
 // FP64 wave tiles of up to 64x32 fit two waves per SIMD without spilling;
 // 64x64 ones need the whole register file of one.
-__global__ __launch_bounds__(WM *WN *KW * 64, (sizeof(T) == 8 && (BM / WM) * (BN / WN) <= 2048) ? 2 : 1)
+__global__ __launch_bounds__(WM *WN *KW * 64, (sizeof(T) == 8 && KW == 1 && (BM / WM) * (BN / WN) <= 2048) ? 2 : 1)
 void gemm_mfma_kernel(GemmParams<T> p) {
-    static_assert(KW == 1 || (sizeof(T) == 4 && BK % (8 * KW) == 0), "k-grouped waves are FP32 only");
+    static_assert(KW == 1 || (sizeof(T) != 2 && BK % (8 * KW) == 0), "k-grouped waves: FP32 and FP64");
     constexpr int NT = WM * WN * KW * 64;
     constexpr int WTM = BM / WM, WTN = BN / WN;
     using Lay = SlabLayout<T, TA, TB, BM, BN, BK>;
     using LA = TileLoader<T, BM, BK, NT, TA, VEC, Lay::KMA>;   // op(A) rows = M; k-contiguous iff A is transposed
     using LB = TileLoader<T, BN, BK, NT, !TB, VEC, Lay::KMB>;  // op(B) rows = N; k-contiguous iff B is not transposed
-    using Wave = std::conditional_t<KW == 1, typename Lay::template Wave<WTM, WTN>,
-                                    WaveF32<WTM, WTN, BK / KW, Lay::PITCH>>;
+    using WaveKW = std::conditional_t<sizeof(T) == 8,
+                                      WaveF64<WTM, WTN, BK / KW, Lay::KMA ? BM + 4 : 0, Lay::KMB ? BN + 4 : 0, Lay::PITCH>,
+                                      WaveF32<WTM, WTN, BK / KW, Lay::PITCH>>;
+    using Wave = std::conditional_t<KW == 1, typename Lay::template Wave<WTM, WTN>, WaveKW>;
     constexpr int STAGE = Lay::STAGE;
     constexpr int BOFF = LA::SIZE;
 
@@ -181,30 +183,32 @@ void gemm_mfma_kernel(GemmParams<T> p) {
     }
 
     if constexpr (KW > 1) {
-        constexpr int PER_WAVE = Wave::SM * Wave::SN * 16 * 64;   // accumulators of one wave
+        // accumulators of one wave (FP32: 16 per sub-tile, FP64: 4 rotations)
+        constexpr int PER_SUB = sizeof(T) == 8 ? 4 : 16;
+        constexpr int PER_WAVE = Wave::SM * Wave::SN * PER_SUB * 64;
         static_assert((KW - 1) * WM * WN * PER_WAVE <= 2 * STAGE, "partials fit the slab buffers");
-        float *red = reinterpret_cast<float *>(smem_raw);
+        T *red = smem;
         __syncthreads();   // every wave is done with the slabs
         if (kg > 0) {
-            float *dst = red + ((kg - 1) * WM * WN + wmn) * PER_WAVE + lane;
+            T *dst = red + ((kg - 1) * WM * WN + wmn) * PER_WAVE + lane;
 #pragma unroll
             for (int sm = 0; sm < Wave::SM; ++sm)
 #pragma unroll
                 for (int sn = 0; sn < Wave::SN; ++sn)
 #pragma unroll
-                    for (int e = 0; e < 16; ++e) dst[((sm * Wave::SN + sn) * 16 + e) * 64] = w.acc[sm][sn][e];
+                    for (int e = 0; e < PER_SUB; ++e) dst[((sm * Wave::SN + sn) * PER_SUB + e) * 64] = w.acc[sm][sn][e];
         }
         __syncthreads();
         if (kg > 0) return;
 #pragma unroll
         for (int g = 1; g < KW; ++g) {
-            const float *src = red + ((g - 1) * WM * WN + wmn) * PER_WAVE + lane;
+            const T *src = red + ((g - 1) * WM * WN + wmn) * PER_WAVE + lane;
 #pragma unroll
             for (int sm = 0; sm < Wave::SM; ++sm)
 #pragma unroll
                 for (int sn = 0; sn < Wave::SN; ++sn)
 #pragma unroll
-                    for (int e = 0; e < 16; ++e) w.acc[sm][sn][e] += src[((sm * Wave::SN + sn) * 16 + e) * 64];
+                    for (int e = 0; e < PER_SUB; ++e) w.acc[sm][sn][e] += src[((sm * Wave::SN + sn) * PER_SUB + e) * 64];
         }
     }
 
@@ -325,24 +329,36 @@ static hipError_t launch_one(const GemmParams<T> &p, int splits, hipStream_t s) 
     return launch_one_as<T, TA, TB, BM, BN, WM, WN, BK, VEC, false>(p, splits, s);
 }
 
-// FP32 64x64 grids of at most one workgroup per CU with partitions of at
-// least 512 k take the k-grouped instance (1024^3: 27.9 -> 24.6 us; shallower
-// partitions measured slower)
+// Grids of at most one workgroup per CU with partitions of at least 512 k
+// take the k-grouped instances where the library chose the configuration:
+// FP32 64x64 with four groups (SGEMM 1024^3 27.9 -> 24.6 us; shallower
+// partitions measured slower), FP64 64x64 and 128x64 with two groups on the
+// fast-epilogue instance, partitions of at most 2048 k (DGEMM 1024^3 58.5 ->
+// 49.5 us; four FP64 groups spill at 1024 threads).  The DTRSM and LU
+// updates keep one group: their solves measured 2% slower with two.
 constexpr int KGROUP_MIN_K = 512;
+template <typename T>
+static bool use_kgroups(const GemmParams<T> &p, int splits, bool kgroups) {
+    return kgroups && p.kchunk >= KGROUP_MIN_K && (sizeof(T) == 4 || p.kchunk <= GEMM_FEPI_MAX_K) &&
+           (int64_t)p.tiles_m * p.tiles_n * splits <= rt().cus;
+}
 
 template <typename T, bool TA, bool TB, bool VEC>
 static hipError_t launch_tile(int tile, const GemmParams<T> &p, int splits, bool kgroups, hipStream_t s) {
     switch (tile) {
     case TILE_128x128: return launch_one<T, TA, TB, 128, 128, 2, 2, 32, VEC>(p, splits, s);
     case TILE_64x64:
-        if constexpr (sizeof(T) == 4) {
-            // four k-groups of four waves on 128-deep slabs: 16 waves per CU
-            // instead of 4
-            if (kgroups && p.kchunk >= KGROUP_MIN_K && (int64_t)p.tiles_m * p.tiles_n * splits <= rt().cus)
-                return launch_one_as<T, TA, TB, 64, 64, 2, 2, 128, VEC, false, 4>(p, splits, s);
+        if (use_kgroups(p, splits, kgroups)) {
+            // FP32: four k-groups of four waves on 128-deep slabs (16 waves
+            // per CU instead of 4); FP64: two groups on 32-deep slabs
+            if constexpr (sizeof(T) == 8) return launch_one_as<T, TA, TB, 64, 64, 2, 2, 32, VEC, true, 2>(p, splits, s);
+            else return launch_one_as<T, TA, TB, 64, 64, 2, 2, 128, VEC, false, 4>(p, splits, s);
         }
         return launch_one<T, TA, TB, 64, 64, 2, 2, sizeof(T) == 8 ? 16 : 32, VEC>(p, splits, s);
-    case TILE_128x64: return launch_one<T, TA, TB, 128, 64, 2, 2, sizeof(T) == 8 ? 16 : 32, VEC>(p, splits, s);
+    case TILE_128x64:
+        if constexpr (sizeof(T) == 8)
+            if (use_kgroups(p, splits, kgroups)) return launch_one_as<T, TA, TB, 128, 64, 2, 2, 32, VEC, true, 2>(p, splits, s);
+        return launch_one<T, TA, TB, 128, 64, 2, 2, sizeof(T) == 8 ? 16 : 32, VEC>(p, splits, s);
     case TILE_LARGE:
         if constexpr (sizeof(T) == 8) return launch_one<T, TA, TB, 256, 64, 4, 1, 16, VEC>(p, splits, s);
         else return launch_one<T, TA, TB, 256, 128, 4, 2, 32, VEC>(p, splits, s);

@@ -221,28 +221,30 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t operand_rsrc(const void *p) {
 // fragment; the two k of a k pair are two such reads.  BPK > 0 does the same
 // for a k-major B slab: column pairs of a 32-column band feed the band's even
 // and odd sub-tiles (the rotation picks the pair).
-template <int WTM, int WTN, int BK, int APK = 0, int BPK = 0>
+// RP is the row pitch of a [row][k] slab and koff the first k of the wave's
+// part (a slab deeper than BK shared by several waves along k).
+template <int WTM, int WTN, int BK, int APK = 0, int BPK = 0, int RP = BK + 2>
 struct WaveF64 {
     static constexpr int SM = WTM / 16, SN = WTN / 16;
-    static constexpr int PITCH = BK + 2;
+    static constexpr int PITCH = RP;
     static_assert(BK % 8 == 0, "FP64 slab depth must be a multiple of 8");
     static_assert(APK == 0 || SM % 2 == 0, "row pairs need an even number of A sub-tiles");
     static_assert(BPK == 0 || SN % 2 == 0, "column pairs need an even number of B sub-tiles");
     double acc[SM][SN][4];
     int aoff[SM], boff[SN][4];
 
-    __device__ __forceinline__ void init(int lane, int wm0, int wn0) {
+    __device__ __forceinline__ void init(int lane, int wm0, int wn0, int koff = 0) {
         const int q = lane & 15, g = lane >> 4, b = q >> 2, j = q & 3;
 #pragma unroll
         for (int sm = 0; sm < SM; ++sm)
-            aoff[sm] = APK ? (BK / 4) * g * APK + wm0 + 32 * (sm >> 1) + 2 * q
-                           : (wm0 + sm * 16 + q) * PITCH + (BK / 4) * g;
+            aoff[sm] = APK ? (koff + (BK / 4) * g) * APK + wm0 + 32 * (sm >> 1) + 2 * q
+                           : (wm0 + sm * 16 + q) * PITCH + koff + (BK / 4) * g;
 #pragma unroll
         for (int sn = 0; sn < SN; ++sn)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                boff[sn][r] = BPK ? (BK / 4) * g * BPK + wn0 + 32 * (sn >> 1) + 2 * (4 * ((b + r) & 3) + j)
-                                  : (wn0 + sn * 16 + 4 * ((b + r) & 3) + j) * PITCH + (BK / 4) * g;
+                boff[sn][r] = BPK ? (koff + (BK / 4) * g) * BPK + wn0 + 32 * (sn >> 1) + 2 * (4 * ((b + r) & 3) + j)
+                                  : (wn0 + sn * 16 + 4 * ((b + r) & 3) + j) * PITCH + koff + (BK / 4) * g;
 #pragma unroll
         for (int sm = 0; sm < SM; ++sm)
 #pragma unroll

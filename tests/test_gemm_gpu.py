@@ -59,24 +59,28 @@ def test_gemm_splitk(dtype, splitk):
     check_vs_oracle(dtype, "N", "T", 70, 90, 1500, 1.25, -0.5, seed=splitk, tile=1, splitk=splitk)
 
 
+@pytest.mark.parametrize("dtype,tile", [(np.float32, 1), (np.float64, 1), (np.float64, 2)],
+                         ids=["s-64x64", "d-64x64", "d-128x64"])
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
-def test_sgemm_k_grouped_waves(ta, tb):
-    """An FP32 64x64 grid of at most one workgroup per CU, with at least 512 k per partition, splits each
-    slab's k over four groups of waves when the library picks the configuration itself, and group 0 adds
-    the other groups' accumulators.  A sweep-table entry makes the library pick 64x64 without split-K
-    for these shapes: ragged edges, K not a multiple of the 128-deep slab, a single tile."""
+def test_gemm_k_grouped_waves(dtype, tile, ta, tb):
+    """A grid of at most one workgroup per CU, with at least 512 k per partition, splits each slab's k
+    over groups of waves (FP32 64x64: four; FP64 64x64 and 128x64: two, up to 2048 k) when the library
+    picks the configuration itself, and group 0 adds the other groups' accumulators.  A sweep-table
+    entry makes the library pick the tile without split-K for these shapes: ragged edges, K not a
+    multiple of the slab, a single tile."""
     from gemm_avx512f_amd import tuning
+    op = 0 if dtype == np.float32 else 1
     shapes = [(1000, 520, 640), (333, 1000, 1100), (64, 64, 512)]
     shp = np.zeros(len(shapes), dtype=tuning.SHAPE_DTYPE)
     ms = np.full((len(shapes), 21), -1.0, dtype=np.float32)
     for i, (m, n, k) in enumerate(shapes):
-        shp[i] = (0, ta == "T", tb == "T", m, n, k, 0)
-        ms[i, tuning.candidate_index(1, 1)] = 1.0
+        shp[i] = (op, ta == "T", tb == "T", m, n, k, 0)
+        ms[i, tuning.candidate_index(tile, 1)] = 1.0
     tuning.table_set(shp, ms)
     try:
         for i, (m, n, k) in enumerate(shapes):
-            assert tuning.pick_config(0, ta == "T", tb == "T", m, n, k, 0, mode=2) == (1, 1)
-            check_vs_oracle(np.float32, ta, tb, m, n, k, 1.5, 0.5, seed=300 + i)
+            assert tuning.pick_config(op, ta == "T", tb == "T", m, n, k, 0, mode=2) == (tile, 1)
+            check_vs_oracle(dtype, ta, tb, m, n, k, 1.5, 0.5, seed=300 + i)
     finally:
         tuning.table_clear()
 
