@@ -164,7 +164,9 @@ int gemm_amd_dgemm(char transa, char transb, int m, int n, int k, double alpha,
 
 /* Same products with an explicit kernel configuration (tile id 0..3 =
  * 128x128, 64x64, 128x64, and 256x64 (FP64) or 256x128 (FP32); splitk >= 1), bypassing the picker.
- * Used by the autotuner; the result is identical up to summation order. */
+ * Used by the autotuner; the result is identical up to summation order.  These always run the one-group
+ * kernel of the named tile (the calls above may split each slab's k over groups of waves on grids of at most
+ * one workgroup per CU). */
 int gemm_amd_sgemm_cfg(char transa, char transb, int m, int n, int k, float alpha,
                        const float *a, int lda, const float *b, int ldb, float beta,
                        float *c, int ldc, int tile, int splitk, void *stream);
