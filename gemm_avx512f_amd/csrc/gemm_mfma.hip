@@ -245,18 +245,21 @@ void gemm_mfma_kernel(GemmParams<T> p) {
     }
 }
 
-// Sum split-K partials in partition order, then apply alpha/beta.
+// Sum split-K partials in partition order, then apply alpha/beta.  Grid
+// (row blocks, columns): rows and columns come from the indices, not from a
+// 64-bit division of a flat index.
 template <typename T>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(int M, int N, int splits, const T *__restrict__ work,
                                                              T alpha, T beta, T *__restrict__ C, int64_t ldc) {
     const int64_t total = (int64_t)M * N;
-    for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * 256) {
-        T s = work[idx];
-        for (int z = 1; z < splits; ++z) s += work[idx + (int64_t)z * total];
-        const int i = (int)(idx % M);
-        const int64_t j = idx / M;
-        T *dst = C + i + j * ldc;
-        *dst = axpby(alpha, s, beta, dst);
+    for (int j = blockIdx.y; j < N; j += gridDim.y) {
+        const T *wj = work + (int64_t)j * M;
+        T *cj = C + (int64_t)j * ldc;
+        for (int i = blockIdx.x * 256 + threadIdx.x; i < M; i += gridDim.x * 256) {
+            T s = wj[i];
+            for (int z = 1; z < splits; ++z) s += wj[i + (int64_t)z * total];
+            cj[i] = axpby(alpha, s, beta, cj + i);
+        }
     }
 }
 
@@ -413,11 +416,11 @@ static hipError_t launch_gemm_t(bool ta, bool tb, int m, int n, int k, T alpha, 
         }
     }
     if (e != hipSuccess || splits == 1) return e;
-    const int64_t total = (int64_t)m * n;
-    int blocks = (int)((total + 255) / 256);
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(splitk_reduce_kernel<T>, dim3(blocks), dim3(256), 0, s, m, n, splits,
-                       static_cast<const T *>(work), alpha, beta, C, ldc);
+    // one 256-row block per workgroup column, up to 65535 columns per launch
+    // row (the kernel strides over the rest)
+    const dim3 grid((m + 255) / 256 < 65535 ? (m + 255) / 256 : 65535, n < 65535 ? n : 65535);
+    hipLaunchKernelGGL(splitk_reduce_kernel<T>, grid, dim3(256), 0, s, m, n, splits, static_cast<const T *>(work),
+                       alpha, beta, C, ldc);
     return hipGetLastError();
 }
 
