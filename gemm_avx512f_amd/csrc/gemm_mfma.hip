@@ -335,15 +335,17 @@ static hipError_t launch_one(const GemmParams<T> &p, int splits, hipStream_t s) 
 // Grids of at most one workgroup per CU with partitions of at least 512 k
 // take the k-grouped instances where the library chose the configuration:
 // FP32 64x64 with four groups (SGEMM 1024^3 27.9 -> 24.6 us; shallower
-// partitions measured slower), FP64 64x64 and 128x64 with two groups on the
+// partitions measured slower), and with two groups on 64-deep slabs for
+// grids of up to two workgroups per CU (1-5% faster on 1024 x 2048 x 1024,
+// 2048 x 1024 x 2048 and the like), FP64 64x64 and 128x64 with two groups on the
 // fast-epilogue instance, partitions of at most 2048 k (DGEMM 1024^3 58.5 ->
 // 49.5 us; four FP64 groups spill at 1024 threads).  The DTRSM and LU
 // updates keep one group: their solves measured 2% slower with two.
 constexpr int KGROUP_MIN_K = 512;
 template <typename T>
-static bool use_kgroups(const GemmParams<T> &p, int splits, bool kgroups) {
+static bool use_kgroups(const GemmParams<T> &p, int splits, bool kgroups, int per_cu = 1) {
     return kgroups && p.kchunk >= KGROUP_MIN_K && (sizeof(T) == 4 || p.kchunk <= GEMM_FEPI_MAX_K) &&
-           (int64_t)p.tiles_m * p.tiles_n * splits <= rt().cus;
+           (int64_t)p.tiles_m * p.tiles_n * splits <= (int64_t)per_cu * rt().cus;
 }
 
 template <typename T, bool TA, bool TB, bool VEC>
@@ -351,6 +353,12 @@ static hipError_t launch_tile(int tile, const GemmParams<T> &p, int splits, bool
     switch (tile) {
     case TILE_128x128: return launch_one<T, TA, TB, 128, 128, 2, 2, 32, VEC>(p, splits, s);
     case TILE_64x64:
+        if constexpr (sizeof(T) == 4)
+            if (use_kgroups(p, splits, kgroups, 2))
+                if ((int64_t)p.tiles_m * p.tiles_n * splits > rt().cus)
+                    // two workgroups per CU: two k-groups on 64-deep slabs
+                    // (69 KB of LDS each), 16 waves per CU instead of 8
+                    return launch_one_as<T, TA, TB, 64, 64, 2, 2, 64, VEC, false, 2>(p, splits, s);
         if (use_kgroups(p, splits, kgroups)) {
             // FP32: four k-groups of four waves on 128-deep slabs (16 waves
             // per CU instead of 4); FP64: two groups on 32-deep slabs

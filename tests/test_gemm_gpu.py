@@ -64,13 +64,16 @@ def test_gemm_splitk(dtype, splitk):
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
 def test_gemm_k_grouped_waves(dtype, tile, ta, tb):
     """A grid of at most one workgroup per CU, with at least 512 k per partition, splits each slab's k
-    over groups of waves (FP32 64x64: four; FP64 64x64 and 128x64: two, up to 2048 k) when the library
+    over groups of waves (FP32 64x64: four, or two for grids of up to two workgroups per CU; FP64 64x64
+    and 128x64: two, up to 2048 k) when the library
     picks the configuration itself, and group 0 adds the other groups' accumulators.  A sweep-table
     entry makes the library pick the tile without split-K for these shapes: ragged edges, K not a
     multiple of the slab, a single tile."""
     from gemm_avx512f_amd import tuning
     op = 0 if dtype == np.float32 else 1
     shapes = [(1000, 520, 640), (333, 1000, 1100), (64, 64, 512)]
+    if dtype == np.float32:
+        shapes.append((1030, 1100, 520))   # 306 tiles: two workgroups per CU, two groups
     shp = np.zeros(len(shapes), dtype=tuning.SHAPE_DTYPE)
     ms = np.full((len(shapes), 21), -1.0, dtype=np.float32)
     for i, (m, n, k) in enumerate(shapes):
